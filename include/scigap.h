@@ -1,0 +1,187 @@
+/*
+ * scigap.h -- C ABI of the MI355X (gfx950) GAP-TV / ADMM-TV library.
+ *
+ * The reference (Scientific-Research-Resource/SCI-algorithms, PnP_SCI/python)
+ * is pure Python/NumPy and has no FFI; each entry point below replaces one
+ * Python function or one loop of it, cited as file:line relative to the
+ * reference checkout.  The Python mirror of the reference interface
+ * (sci-algorithms_amd/pnp_sci) binds these with ctypes.
+ *
+ * Conventions
+ *  - All array pointers are DEVICE pointers owned by the caller, fp32.
+ *  - Video volumes are frame-major (T,H,W), C order: element (t,h,w) is at
+ *    t*H*W + h*W + w.  The reference holds (H,W,T); scigap_hwt_to_thw /
+ *    scigap_thw_to_hwt convert on the device.
+ *  - Planes (measurement y, Phi_sum, ...) are (H,W), C order.
+ *  - `stream` is a hipStream_t (NULL = the null stream).  Calls are
+ *    asynchronous on that stream unless documented otherwise; calls on
+ *    different streams with disjoint buffers may run concurrently.
+ *  - Every entry point returns 0 on success or a negative SCIGAP_E* code;
+ *    scigap_last_error() then returns a message (per host thread).  No C++
+ *    exception crosses this boundary.
+ *  - Scratch memory is a caller-provided device workspace whose size the
+ *    matching *_workspace_bytes() function returns; the library allocates
+ *    no device memory on these paths.
+ */
+#ifndef SCIGAP_H
+#define SCIGAP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCIGAP_OK            0
+#define SCIGAP_EINVAL       -1  /* bad shape, size or parameter */
+#define SCIGAP_ENULL        -2  /* required pointer is NULL */
+#define SCIGAP_EHIP         -3  /* HIP runtime error (message has details) */
+#define SCIGAP_EWORKSPACE   -4  /* workspace too small */
+#define SCIGAP_EALIAS       -5  /* input and output buffers overlap */
+#define SCIGAP_EUNSUPPORTED -6  /* parameter combination not implemented */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int scigap_version(void);
+
+/* Message describing the last error on this host thread ("" if none). */
+const char* scigap_last_error(void);
+
+/* ---------------------------------------------------------------------- */
+/* Forward model                                                          */
+/* ---------------------------------------------------------------------- */
+
+/* y[h,w] = sum_t x[t,h,w] * phi[t,h,w].
+ * Replaces A_(x, Phi), PnP_SCI/python/utils.py:10-15. */
+int scigap_forward(const float* x, const float* phi, float* y,
+                   int64_t H, int64_t W, int64_t T, void* stream);
+
+/* x[t,h,w] = y[h,w] * phi[t,h,w].
+ * Replaces At_(y, Phi), PnP_SCI/python/utils.py:17-26. */
+int scigap_adjoint(const float* y, const float* phi, float* x,
+                   int64_t H, int64_t W, int64_t T, void* stream);
+
+/* phi_sum[h,w] = sum_t phi[t,h,w], with exact zeros replaced by 1.
+ * Replaces pnp_sci_algo.py:491-492 (and pnp_sci_demo_kobe.py:77-78). */
+int scigap_phi_sum(const float* phi, float* phi_sum,
+                   int64_t H, int64_t W, int64_t T, void* stream);
+
+/* Layout conversion between the reference's (H,W,T) and the library's
+ * (T,H,W).  src and dst must not overlap.  No reference counterpart: the
+ * reference never leaves NumPy's (H,W,T). */
+int scigap_hwt_to_thw(const float* src, float* dst,
+                      int64_t H, int64_t W, int64_t T, void* stream);
+int scigap_thw_to_hwt(const float* src, float* dst,
+                      int64_t H, int64_t W, int64_t T, void* stream);
+
+/* sum over all elements of (a-b)^2, accumulated in fp64 into *out (device
+ * pointer, overwritten).  Used for PSNR, utils.py:28-36 and
+ * pnp_sci_algo.py:681-682,699-705. per_frame != 0 writes T values (one per
+ * frame of a (T,H,W) volume) instead of one. */
+int scigap_sq_error(const float* a, const float* b, double* out,
+                    int64_t H, int64_t W, int64_t T, int per_frame,
+                    void* stream);
+
+/* Mean structural similarity of every frame of two (T,H,W) volumes, as
+ * scikit-image 0.18.3 structural_similarity(X, Y, data_range) with its
+ * defaults (7x7 uniform window, sample covariance, K1=0.01, K2=0.03, mean
+ * over the image minus a 3-pixel border), called per frame by the reference
+ * at pnp_sci_algo.py:703 and :861.  Writes T doubles to `out` (device).
+ * Requires H >= 7 and W >= 7. */
+int scigap_ssim(const float* a, const float* b, double* out,
+                int64_t H, int64_t W, int64_t T, float data_range, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Chambolle TV                                                           */
+/* ---------------------------------------------------------------------- */
+
+/* Per-frame 2-D Chambolle TV of a (T,H,W) volume, as scikit-image 0.18.3
+ * denoise_tv_chambolle(image, weight, eps, n_iter_max, multichannel=True)
+ * (skimage/restoration/_denoise.py:315-479), called by the reference at
+ * pnp_sci_algo.py:650 and :812: the dual field starts at zero, n_iter_max
+ * iterations perform n_iter_max-1 dual updates, and each frame stops early
+ * when |E_prev - E| < eps*E_0.  `image` and `out` must not overlap.
+ * `stop_iter` (optional, device, T ints) receives the iteration whose image
+ * was returned per frame. */
+size_t scigap_tv_workspace_bytes(int64_t H, int64_t W, int64_t T, int n_iter_max);
+int scigap_tv_chambolle(const float* image, float* out,
+                        int64_t H, int64_t W, int64_t T,
+                        float weight, float eps, int n_iter_max,
+                        int* stop_iter, void* workspace, size_t workspace_bytes,
+                        void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* GAP-TV / ADMM-TV solver                                                */
+/* ---------------------------------------------------------------------- */
+
+#define SCIGAP_METHOD_GAP  0   /* pnp_sci_algo.py:536-706            */
+#define SCIGAP_METHOD_ADMM 1   /* pnp_sci_algo.py:708-864            */
+
+typedef struct scigap_params {
+  int64_t H, W, T;     /* measurement H x W, T coded frames              */
+  int method;          /* SCIGAP_METHOD_GAP or SCIGAP_METHOD_ADMM        */
+  int accelerate;      /* GAP: accelerated update (pnp_sci_algo.py:641)  */
+  int clip;            /* ADMM: clip theta to [0,1] after TV
+                          (joint_pnp_sci_algo.py:633)                    */
+  int iters;           /* outer iterations (sum of iter_max stages)      */
+  float lambda_;       /* _lambda                                        */
+  float gamma;         /* ADMM gamma (added to Phi_sum)                  */
+  float tv_weight;     /* denoise_tv_chambolle weight                    */
+  float tv_eps;        /* denoise_tv_chambolle eps (2e-4 by default)     */
+  int tv_iter_max;     /* denoise_tv_chambolle n_iter_max (>= 1)         */
+  float tv_weight_decay; /* per-iteration factor on tv_weight (1: none;
+                            0.999 in ADMM_TV_rec, pnp_sci_algo.py:898)   */
+  float gamma_decay;   /* per-iteration factor on gamma (1: none;
+                          0.998 in ADMM_TV_rec, pnp_sci_algo.py:899)     */
+} scigap_params;
+
+/* Fill *p with the reference defaults for a GAP solve of an HxWxT problem:
+ * accelerate=1, lambda=1, gamma=0.01, tv_weight=0.1, tv_eps=2e-4,
+ * tv_iter_max=5, iters=50, no decay (pnp_sci_algo.py:536-539,708-711). */
+void scigap_default_params(scigap_params* p, int64_t H, int64_t W, int64_t T);
+
+size_t scigap_solve_workspace_bytes(const scigap_params* p);
+
+/* One full GAP-TV or ADMM-TV reconstruction.
+ *   y        (H,W)   measurement
+ *   phi      (T,H,W) masks
+ *   phi_sum  (H,W)   taken as passed (the reference does not recompute it)
+ *   x0       (T,H,W) warm start, or NULL for x0 = At(y) (pnp_sci_algo.py:625-627)
+ *   x_out    (T,H,W) result: GAP returns x after the last TV step, ADMM
+ *                    returns the last projection x (pnp_sci_algo.py:706,864)
+ *   truth    (T,H,W) or NULL; when given, sq_err[k] receives
+ *                    sum((truth - x_k)^2) in fp64 for every outer iteration
+ *                    k (device array of `iters` doubles), the quantity behind
+ *                    psnr_all (pnp_sci_algo.py:681-682)
+ *   stats    optional device array of `iters` ints: per iteration, the
+ *            number of frames whose TV stopped early on the energy test.
+ * x0 may not overlap x_out.  The call is asynchronous on `stream`. */
+int scigap_solve(const scigap_params* p,
+                 const float* y, const float* phi, const float* phi_sum,
+                 const float* x0, float* x_out,
+                 const float* truth, double* sq_err, int* stats,
+                 void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Kernel timing (measurement support; no reference counterpart)          */
+/* ---------------------------------------------------------------------- */
+
+typedef struct scigap_kernel_stat {
+  char name[48];     /* kernel kind, e.g. "fused_interior"                */
+  int launches;      /* launches recorded since scigap_profile_begin()     */
+  double total_ms;   /* sum of their durations (HIP events on the stream) */
+} scigap_kernel_stat;
+
+/* Start recording a HIP event pair around every kernel launch the library
+ * makes (process-wide; adds a little launch overhead while enabled). */
+int scigap_profile_begin(void);
+
+/* Stop recording, wait for the recorded events, and write up to `max`
+ * per-kind totals to `out` (host memory); *n receives the number of kinds. */
+int scigap_profile_end(scigap_kernel_stat* out, int max, int* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCIGAP_H */

@@ -1,0 +1,259 @@
+// Register-tiled Chambolle TV engine for gfx950 (device code only).
+//
+// Arithmetic of one dual update, per frame (scikit-image 0.18.3
+// restoration/_denoise.py:349-392, 2-D, tau = 1/4):
+//   d   = p0[r-1,c]*[r>0] - p0[r,c] + p1[r,c-1]*[c>0] - p1[r,c]
+//   out = image + d                      (out = image before the first update)
+//   g0  = out[r+1,c] - out[r,c]  (0 on the last row)
+//   g1  = out[r,c+1] - out[r,c]  (0 on the last column)
+//   |g| = sqrt(g0^2 + g1^2)
+//   p   = (p - tau*g) / (1 + (tau/weight)*|g|)
+//   E   = sum d^2 + weight * sum |g|     (per frame; drives the early exit)
+//
+// Work decomposition.  A workgroup of BY wavefronts owns a load tile of
+// 128 columns x BY*RY rows of one frame (RY = MR*D rows per lane, D = number
+// of fused updates).  Lane l of wave w holds columns c0+2l, c0+2l+1 of rows
+// r0+w*RY .. r0+w*RY+RY-1 in registers (img, out, p0, p1).  Horizontal
+// neighbours come from the adjacent lane through DPP wave_shl / wave_shr;
+// vertical neighbours come from the lane's own registers except at the wave's
+// first/last row, which are exchanged through 2*128 floats of LDS per wave
+// (one barrier per exchange).  Every update invalidates one more ring of the
+// load tile, so after D updates the result is exact on the tile interior:
+// rows [D, BY*RY-D) and columns [HC, 128-HC) with HC = D rounded up to even.
+// Outside the image every gradient is masked to zero, which keeps p = 0 there
+// and reproduces the reference's boundary handling at the image border; those
+// pixels then contribute nothing to the energies.
+#pragma once
+
+#include "scigap_internal.h"
+
+namespace scigap {
+
+template <int D, int BY, int MR>
+struct TileGeom {
+  static constexpr int RY = MR * D;            // rows per lane
+  static constexpr int HC = (D + 1) & ~1;      // column halo (even)
+  static constexpr int LW = 128;               // load-tile width
+  static constexpr int LH = BY * RY;           // load-tile height
+  static constexpr int OX = LW - 2 * HC;       // interior width
+  static constexpr int OY = LH - 2 * D;        // interior height
+  static_assert(MR == 1 || MR == 2, "row multiplier must be 1 or 2");
+  static_assert(BY >= 3, "need at least one interior wave");
+  static_assert(D >= 1 && D <= 8, "fused depth out of range");
+};
+
+// LDS used by the engine: one row of `out` and one row of `p0` per wave.
+template <int BY>
+struct TileLds {
+  float xo[BY * 128];
+  float xp[BY * 128];
+};
+
+// Per-lane view of where its pixels are (for masks and ownership).
+struct LanePos {
+  int row0;   // global row of register row 0 (wave-uniform)
+  int col;    // global column of register column 0
+  int w;      // wave index in the workgroup (wave-uniform)
+  int lane;   // lane index
+};
+
+// Gradient masks for tiles that touch the image border: a forward
+// difference is kept where the reference computes one and forced to zero
+// elsewhere (last row/column, and every pixel outside the image).  Rows are
+// wave-uniform (bit j of rowok0/rowin, held in scalar registers); columns are
+// per lane (two floats per kind).
+struct EdgeMasks {
+  unsigned rowok0 = 0;  // bit j: 0 <= row0+j < H-1   (vertical difference)
+  unsigned rowin = 0;   // bit j: 0 <= row0+j < H     (horizontal difference)
+  float colin[2];       // 0 <= col+k < W             (vertical difference)
+  float colok1[2];      // 0 <= col+k < W-1           (horizontal difference)
+  template <int RY>
+  __device__ void init(const LanePos& lp, int H, int W) {
+    rowok0 = rowin = 0;
+#pragma unroll
+    for (int j = 0; j < RY; ++j) {
+      int r = lp.row0 + j;
+      rowok0 |= unsigned(r >= 0 && r < H - 1) << j;
+      rowin |= unsigned(r >= 0 && r < H) << j;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      int c = lp.col + k;
+      colin[k] = (c >= 0 && c < W) ? 1.f : 0.f;
+      colok1[k] = (c >= 0 && c < W - 1) ? 1.f : 0.f;
+    }
+  }
+};
+
+// Energy partials of one lane are kept in a float[NH*8] array indexed
+// [h*8 + u] (update u, strip half h).  With NH = 2, half 0 covers rows
+// [0, RY/2) and half 1 rows [RY/2, RY) of the lane's strip (needed when a
+// wave holds both halo and interior rows); with NH = 1 one sum per update.
+
+// out = img + d(p), with img read from LDS (row j at img[j*64]); adds sum d^2
+// of each strip half h to dacc[h*8] when dacc is non-null.
+template <int RY, int BY, int NH>
+__device__ __forceinline__ void tv_div(const float2* img, float (&out)[RY][2],
+                                       const float (&p0)[RY][2], const float (&p1)[RY][2],
+                                       TileLds<BY>& lds, const LanePos& lp, float* dacc) {
+  // Publish this wave's last row of p0 for the wave below.
+  float2* xp = reinterpret_cast<float2*>(lds.xp);
+  xp[lp.w * 64 + lp.lane] = make_float2(p0[RY - 1][0], p0[RY - 1][1]);
+  __syncthreads();
+  float2 above = lp.w > 0 ? xp[(lp.w - 1) * 64 + lp.lane] : make_float2(0.f, 0.f);
+  float s[NH] = {};
+#pragma unroll
+  for (int j = 0; j < RY; ++j) {
+    float up0 = j > 0 ? p0[j - 1][0] : above.x;
+    float up1 = j > 0 ? p0[j - 1][1] : above.y;
+    float left0 = from_prev_lane(p1[j][1]);
+    float d0 = (up0 - p0[j][0]) + (left0 - p1[j][0]);
+    float d1 = (up1 - p0[j][1]) + (p1[j][0] - p1[j][1]);
+    const float2 im = img[j * 64];
+    out[j][0] = im.x + d0;
+    out[j][1] = im.y + d1;
+    const int h = (NH == 2 && j >= RY / 2) ? 1 : 0;
+    s[h] = fmaf(d0, d0, s[h]);
+    s[h] = fmaf(d1, d1, s[h]);
+  }
+  if (dacc)
+#pragma unroll
+    for (int h = 0; h < NH; ++h) dacc[h * 8] += s[h];
+}
+
+// One dual update p <- (p - tau g)/(1 + tw |g|) from the current `out`;
+// adds weight*sum|g| of each strip half h to gacc[h*8] when gacc is non-null.
+template <int RY, int BY, int NH, bool EDGE>
+__device__ __forceinline__ void tv_grad(const float (&out)[RY][2], float (&p0)[RY][2],
+                                        float (&p1)[RY][2], TileLds<BY>& lds, const LanePos& lp,
+                                        const EdgeMasks& em, float weight, float tw, float* gacc) {
+  constexpr float tau = 0.25f;
+  float2* xo = reinterpret_cast<float2*>(lds.xo);
+  xo[lp.w * 64 + lp.lane] = make_float2(out[0][0], out[0][1]);
+  __syncthreads();
+  float2 below = lp.w + 1 < BY ? xo[(lp.w + 1) * 64 + lp.lane]
+                               : make_float2(out[RY - 1][0], out[RY - 1][1]);
+  float s[NH] = {};
+#pragma unroll
+  for (int j = 0; j < RY; ++j) {
+    float dn0 = j + 1 < RY ? out[j + 1][0] : below.x;
+    float dn1 = j + 1 < RY ? out[j + 1][1] : below.y;
+    float right1 = from_next_lane(out[j][0]);
+    float g00 = dn0 - out[j][0];
+    float g01 = dn1 - out[j][1];
+    float g10 = out[j][1] - out[j][0];
+    float g11 = right1 - out[j][1];
+    if constexpr (EDGE) {
+      const bool v_ok = (em.rowok0 >> j) & 1u, h_ok = (em.rowin >> j) & 1u;
+      g00 = v_ok ? g00 * em.colin[0] : 0.f;
+      g01 = v_ok ? g01 * em.colin[1] : 0.f;
+      g10 = h_ok ? g10 * em.colok1[0] : 0.f;
+      g11 = h_ok ? g11 * em.colok1[1] : 0.f;
+    }
+    float n0 = fast_sqrt(fmaf(g00, g00, g10 * g10));
+    float n1 = fast_sqrt(fmaf(g01, g01, g11 * g11));
+    const int h = (NH == 2 && j >= RY / 2) ? 1 : 0;
+    s[h] += n0 + n1;
+    float i0 = fast_rcp(fmaf(tw, n0, 1.f));
+    float i1 = fast_rcp(fmaf(tw, n1, 1.f));
+    p0[j][0] = fmaf(-tau, g00, p0[j][0]) * i0;
+    p1[j][0] = fmaf(-tau, g10, p1[j][0]) * i0;
+    p0[j][1] = fmaf(-tau, g01, p0[j][1]) * i1;
+    p1[j][1] = fmaf(-tau, g11, p1[j][1]) * i1;
+    // Materialise the new dual values here; otherwise the compiler folds the
+    // multiply by 1/den into the next divergence and keeps both factors live.
+    asm("" : "+v"(p0[j][0]), "+v"(p1[j][0]), "+v"(p0[j][1]), "+v"(p1[j][1]));
+  }
+  if (gacc)
+#pragma unroll
+    for (int h = 0; h < NH; ++h) gacc[h * 8] = fmaf(weight, s[h], gacc[h * 8]);
+}
+
+// Run `nup` (<= D) updates starting from p = 0 (out = img).  `img` points at
+// this lane's float2 of register row 0 in an LDS plane laid out [row][64 lanes]
+// (row j at img[j*64]).  On return `out` holds the image after nup updates and
+// acc.e[h][u] this lane's partial of E_u over half h of its strip.  When
+// `accumulate` is false the energies are not formed (recomputation passes).
+// Energy slots are addressed as e[h][u] = base[h*8 + u], so D <= 8.
+template <int D, int RY, int BY, int NH, bool EDGE>
+__device__ __forceinline__ void tv_run_from_zero(const float2* img, float (&out)[RY][2], int nup,
+                                                 TileLds<BY>& lds, const LanePos& lp,
+                                                 const EdgeMasks& em, float weight, float tw,
+                                                 float (&acc)[NH * 8], bool accumulate) {
+  float p0[RY][2], p1[RY][2];
+#pragma unroll
+  for (int j = 0; j < RY; ++j) {
+    const float2 im = img[j * 64];
+    out[j][0] = im.x;
+    out[j][1] = im.y;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      p0[j][k] = 0.f;
+      p1[j][k] = 0.f;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NH * 8; ++i) acc[i] = 0.f;
+#pragma unroll
+  for (int u = 0; u < D; ++u) {
+    if (u < nup) {
+      tv_grad<RY, BY, NH, EDGE>(out, p0, p1, lds, lp, em, weight, tw, accumulate ? &acc[u] : nullptr);
+      const bool dn = accumulate && u + 1 < D;
+      tv_div<RY, BY, NH>(img, out, p0, p1, lds, lp, dn ? &acc[u + 1] : nullptr);
+    }
+  }
+}
+
+// Reduce the energy partials over the workgroup and add the totals to
+// energy[0..D) in fp64.  A lane contributes half h of its strip when those
+// rows are interior rows of the tile and its columns are interior columns.
+template <int D, int BY, int MR, int NH>
+__device__ __forceinline__ void energy_commit(const float (&acc)[NH * 8], bool lane_owner,
+                                              double* energy, float* red /* LDS, BY*D */,
+                                              const LanePos& lp) {
+  // Which halves of this wave's strip are interior rows.
+  bool lo_in, hi_in;
+  if (MR == 2) {
+    lo_in = lp.w > 0;          // rows [0, D) of wave 0 are the top halo
+    hi_in = lp.w < BY - 1;     // rows [D, 2D) of the last wave are the bottom halo
+  } else {
+    lo_in = hi_in = lp.w > 0 && lp.w < BY - 1;
+  }
+#pragma unroll
+  for (int u = 0; u < D; ++u) {
+    float v = 0.f;
+    if (lane_owner) {
+      v = lo_in ? acc[u] : 0.f;
+      if (NH == 2) v += hi_in ? acc[8 + u] : 0.f;
+    }
+    v = wave_sum(v);
+    if (lp.lane == 0) red[lp.w * D + u] = v;
+  }
+  __syncthreads();
+  if (lp.w == 0 && lp.lane < D) {
+    double tot = 0.0;
+#pragma unroll
+    for (int q = 0; q < BY; ++q) tot += double(red[q * D + lp.lane]);
+    atomicAdd(energy + lp.lane, tot);
+  }
+  // The next writer of `red` is this function for the next frame, after at
+  // least two more barriers inside the engine.
+}
+
+// Early-exit iteration of one frame from its energies E_0..E_{D-1}
+// (_denoise.py:384-391): the first i in [1, D) with |E_{i-1} - E_i| < eps*E_0,
+// else D (no early exit; the caller returns the image after D updates).
+template <int D>
+__device__ __forceinline__ int exit_iteration(const double* e, double eps) {
+  const double e0 = e[0];
+  double prev = e0;
+#pragma unroll
+  for (int i = 1; i < D; ++i) {
+    double ei = e[i];
+    if (fabs(prev - ei) < eps * e0) return i;
+    prev = ei;
+  }
+  return D;
+}
+
+}  // namespace scigap

@@ -1,0 +1,19 @@
+"""MI355X-native GAP-TV / ADMM-TV reconstruction for video snapshot
+compressive imaging: the ``denoiser='tv'`` path of PnP-SCI
+(PnP_SCI/python/pnp_sci_algo.py, utils.py) on hand-written gfx950 kernels.
+
+Public names mirror the reference modules:
+
+* ``pnp_sci.pnp_sci_algo``: gap_denoise, admm_denoise, admmdenoise_cacti,
+  GAP_TV_rec, ADMM_TV_rec
+* ``pnp_sci.utils``: A_, At_, psnr, make_operators
+* ``pnp_sci.tv``: denoise_tv_chambolle (scikit-image signature)
+"""
+
+from .pnp_sci_algo import (ADMM_TV_rec, GAP_TV_rec, admm_denoise, admmdenoise_cacti,
+                           gap_denoise, solve_device)
+from .tv import denoise_tv_chambolle
+from .utils import A_, At_, make_operators, psnr
+
+__all__ = ["gap_denoise", "admm_denoise", "admmdenoise_cacti", "GAP_TV_rec", "ADMM_TV_rec",
+           "solve_device", "denoise_tv_chambolle", "A_", "At_", "psnr", "make_operators"]

@@ -1,0 +1,228 @@
+"""Pins for the NumPy oracle (CPU only).
+
+The oracle is checked against (1) golden vectors written by scikit-image
+0.18.3 itself (tests/golden/make_tv_golden.py), bit-exact in float32 and
+float64, and (2) properties the mathematics fixes, chosen so that a dropped
+term, a wrong sign or a shifted index fails at least one of them.
+"""
+
+import numpy as np
+import pytest
+import scipy.ndimage as ndi
+
+import pnp_sci_oracle as O
+
+
+def _golden(golden_dir, name):
+    return np.load("%s/%s" % (golden_dir, name), allow_pickle=False)
+
+
+# ---------------------------------------------------------------------------
+# Chambolle TV against scikit-image 0.18.3
+# ---------------------------------------------------------------------------
+
+def test_tv_matches_skimage_golden_bit_exact(golden_dir):
+    z = _golden(golden_dir, "tv_chambolle_golden.npz")
+    assert str(z["skimage_version"]) == "0.18.3"
+    meta = z["meta"]
+    assert len(meta) > 100
+    for k, (w, n, eps) in enumerate(meta):
+        img, want = z["in_%d" % k], z["out_%d" % k]
+        got = O.denoise_tv_chambolle(img, float(w), eps=float(eps), n_iter_max=int(n),
+                                     multichannel=True)
+        assert got.dtype == want.dtype
+        assert np.array_equal(got, want), "case %d (%s, w=%g, n=%d)" % (k, img.shape, w, n)
+
+
+def test_gap_admm_loops_match_golden_bit_exact(golden_dir):
+    g = _golden(golden_dir, "gaptv_small_golden.npz")
+    for k in range(len(g["meta"])):
+        y, m = g["y_%d" % k], g["mask_%d" % k]
+        A = lambda x: O.A_(x, m)  # noqa: E731
+        At = lambda v: O.At_(v, m)  # noqa: E731
+        ps = O.phi_sum(m)
+        runs = {
+            "gap_acc": O.gap_denoise(y, ps, A, At, iter_max=12, tv_weight=0.1, show_iqa=False)[0],
+            "gap_plain": O.gap_denoise(y, ps, A, At, accelerate=False, iter_max=12,
+                                       tv_weight=0.1, show_iqa=False)[0],
+            "admm": O.admm_denoise(y, ps, A, At, iter_max=12, tv_weight=0.1, show_iqa=False)[0],
+            "admm_clip": O.admm_denoise_clip(y, ps, A, At, iter_max=12, tv_weight=0.1,
+                                             show_iqa=False)[0],
+        }
+        for name, got in runs.items():
+            assert np.array_equal(got, g["%s_%d" % (name, k)]), (name, k)
+
+
+def test_tv_early_exit_branch_is_exercised():
+    # A smooth frame stops long before n_iter_max; the golden cases above pin
+    # what is returned, this pins that the branch is reached at all.
+    r = np.arange(48)[:, None]
+    c = np.arange(40)[None, :]
+    img = 0.5 + 0.1 * np.cos(2 * np.pi * (r / 48.0 + c / 80.0))
+    it = {}
+    out = O.tv_chambolle_nd(img, 0.1, 2e-4, 200, _stats=it)
+    assert 1 <= it["returned"] < 199
+    # The same frame with a tiny eps runs to the end.
+    it2 = {}
+    O.tv_chambolle_nd(img, 0.1, 1e-30, 12, _stats=it2)
+    assert it2["returned"] == 11
+    assert out.shape == img.shape
+
+
+# ---------------------------------------------------------------------------
+# Chambolle TV: properties
+# ---------------------------------------------------------------------------
+
+def test_tv_one_iteration_is_identity():
+    img = np.random.default_rng(0).random((9, 7))
+    assert np.array_equal(O.tv_chambolle_nd(img, 0.3, n_iter_max=1), img)
+
+
+def test_tv_constant_frame_unchanged():
+    img = np.full((6, 5), 0.37)
+    assert np.array_equal(O.tv_chambolle_nd(img, 0.2, n_iter_max=7), img)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 20])
+def test_tv_preserves_frame_mean(n):
+    img = np.random.default_rng(n).random((13, 11))
+    out = O.tv_chambolle_nd(img, 0.2, n_iter_max=n)
+    assert abs(out.mean() - img.mean()) < 1e-14
+
+
+def test_tv_large_weight_is_neumann_heat_step():
+    # n=2 performs one dual step p = -tau*g/(1+(tau/w)|g|); for w -> inf the
+    # result is img + tau*Laplacian(img) with Neumann (replicated) borders,
+    # computed independently by scipy.ndimage.laplace.
+    img = np.random.default_rng(1).random((12, 17))
+    got = O.tv_chambolle_nd(img, 1e12, n_iter_max=2)
+    want = img + 0.25 * ndi.laplace(img, mode="nearest")
+    assert np.max(np.abs(got - want)) < 1e-10
+
+
+def test_tv_frames_independent_and_tv_decreases():
+    rng = np.random.default_rng(2)
+    vol = rng.random((20, 18, 3))
+    out = O.denoise_tv_chambolle(vol, 0.1, n_iter_max=5, multichannel=True)
+    for c in range(3):
+        assert np.array_equal(out[..., c], O.tv_chambolle_nd(vol[..., c], 0.1, n_iter_max=5))
+
+    def tv(a):
+        return np.abs(np.diff(a, axis=0)).sum() + np.abs(np.diff(a, axis=1)).sum()
+
+    for c in range(3):
+        assert tv(out[..., c]) < tv(vol[..., c])
+        assert out[..., c].std() < vol[..., c].std()
+
+
+# ---------------------------------------------------------------------------
+# Forward model and solvers: properties
+# ---------------------------------------------------------------------------
+
+def _problem(H=10, W=9, T=4, seed=0, p=0.5):
+    rng = np.random.default_rng(seed)
+    mask = (rng.random((H, W, T)) < p).astype(np.float64)
+    x = rng.random((H, W, T))
+    return mask, x, rng
+
+
+def test_forward_adjoint_pair():
+    mask, x, rng = _problem()
+    y = rng.random(mask.shape[:2])
+    assert np.isclose(np.sum(O.A_(x, mask) * y), np.sum(x * O.At_(y, mask)), rtol=1e-13)
+    assert np.array_equal(O.At_(np.ones(mask.shape[:2]), mask), mask)
+    assert np.allclose(O.A_(O.At_(y, mask), mask), y * (mask ** 2).sum(axis=2), rtol=1e-14)
+
+
+def test_phi_sum_zero_fix():
+    mask, _, _ = _problem()
+    mask[3, 4, :] = 0
+    s = O.phi_sum(mask)
+    assert s[3, 4] == 1
+    assert np.array_equal(s[mask.sum(2) > 0], mask.sum(2)[mask.sum(2) > 0])
+
+
+def test_plain_gap_identity_denoiser_is_exact_projection(capsys):
+    mask, x, _ = _problem(T=5)
+    mask[2, 2, :] = 0  # a pixel no frame sees: Phi_sum fixed to 1, y = 0 there
+    y = O.A_(x, mask)
+    ps = O.phi_sum(mask)
+    xr, _, _, _ = O.gap_denoise(y, ps, lambda v: O.A_(v, mask), lambda v: O.At_(v, mask),
+                                accelerate=False, iter_max=1, tvm="none", show_iqa=False)
+    assert np.allclose(O.A_(xr, mask), y, atol=1e-12)
+    assert "no such tv denoiser" in capsys.readouterr().out
+
+
+def test_accelerated_first_step_from_zero_equals_plain_step():
+    mask, x, _ = _problem(T=6, seed=3)
+    y = O.A_(x, mask)
+    ps = O.phi_sum(mask)
+    A = lambda v: O.A_(v, mask)  # noqa: E731
+    At = lambda v: O.At_(v, mask)  # noqa: E731
+    x0 = np.zeros_like(x)
+    a = O.gap_denoise(y, ps, A, At, accelerate=True, iter_max=1, tv_iter_max=3, x0=x0,
+                      show_iqa=False)[0]
+    b = O.gap_denoise(y, ps, A, At, accelerate=False, iter_max=1, tv_iter_max=3, x0=x0,
+                      show_iqa=False)[0]
+    assert np.array_equal(a, b)
+    c = O.gap_denoise(y, ps, A, At, accelerate=True, iter_max=2, tv_iter_max=3, x0=x0,
+                      show_iqa=False)[0]
+    d = O.gap_denoise(y, ps, A, At, accelerate=False, iter_max=2, tv_iter_max=3, x0=x0,
+                      show_iqa=False)[0]
+    assert not np.array_equal(c, d)  # the y1 memory matters from step 2 on
+
+
+def test_admm_gamma0_identity_equals_plain_gap():
+    mask, x, _ = _problem(T=4, seed=4)
+    y = O.A_(x, mask)
+    ps = O.phi_sum(mask)
+    A = lambda v: O.A_(v, mask)  # noqa: E731
+    At = lambda v: O.At_(v, mask)  # noqa: E731
+    a = O.admm_denoise(y, ps, A, At, gamma=0.0, iter_max=6, tv_iter_max=1, show_iqa=False)[0]
+    b = O.gap_denoise(y, ps, A, At, accelerate=False, iter_max=6, tv_iter_max=1,
+                      show_iqa=False)[0]
+    assert np.array_equal(a, b)
+
+
+def test_gap_psnr_trajectory_and_errors():
+    mask, x, _ = _problem(H=16, W=16, T=4, seed=5)
+    y = O.A_(x, mask)
+    ps = O.phi_sum(mask)
+    A = lambda v: O.A_(v, mask)  # noqa: E731
+    At = lambda v: O.At_(v, mask)  # noqa: E731
+    xr, p_, _, pall = O.gap_denoise(y, ps, A, At, iter_max=[3, 2], sigma=[None, None],
+                                    X_orig=x, show_iqa=True)
+    assert len(pall) == 5 and len(p_) == 4
+    assert pall[-1] == pytest.approx(O.psnr(x, xr))
+    with pytest.raises(ValueError):
+        O.gap_denoise(y, ps, A, At, denoiser="bm3d")
+    with pytest.raises(NameError):
+        O.gap_denoise(y, ps, A, At, tvm="ITV2D_cham", iter_max=1)
+    assert O.psnr(x, x) == 100
+
+
+def test_cacti_driver_flips_and_normalises():
+    rng = np.random.default_rng(6)
+    H, W, T, F = 12, 10, 4, 2
+    mask = (rng.random((H, W, T)) < 0.5).astype(np.float32)
+    orig = rng.random((H, W, T * F)).astype(np.float32) * 255
+    meas = np.stack([O.A_(orig[..., f * T:(f + 1) * T], mask) for f in range(F)], -1)
+    A = lambda v: O.A_(v, mask)  # noqa: E731
+    At = lambda v: O.At_(v, mask)  # noqa: E731
+    args = dict(denoiser="tv", iter_max=3, tv_weight=0.1, tv_iter_max=3, show_iqa=False)
+    x_, _, p_, _, pall = O.admmdenoise_cacti(meas, mask, A, At, projmeth="gap", orig=orig,
+                                             nframe=F, MAXB=255., maskdirection="updown", **args)
+    assert x_.shape == (H, W, T * F) and x_.dtype == np.float32
+    # Frame block 0 is 'up' (no flip): identical to a direct solve.
+    direct = O.gap_denoise(meas[..., 0] / 255., O.phi_sum(mask), A, At, **args)[0]
+    assert np.allclose(x_[..., :T], direct, rtol=1e-6)
+    assert len(p_) == T * F and len(pall) == F
+
+
+def test_ssim_matches_skimage_golden(golden_dir):
+    z = _golden(golden_dir, "ssim_golden.npz")
+    for k, want in enumerate(z["ssim"]):
+        got = O.compare_ssim(z["x_%d" % k], z["y_%d" % k], data_range=1.)
+        assert abs(got - want) < 1e-12, (k, got, want)
+    with pytest.raises(ValueError):
+        O.compare_ssim(np.zeros((6, 9)), np.zeros((6, 9)))
