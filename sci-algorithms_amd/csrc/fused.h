@@ -1,0 +1,640 @@
+// Fused GAP/ADMM-TV device code shared by the per-solve (gap_fused.hip) and
+// batched (batch.hip) paths: the launch arguments, the fused tile body, the
+// early-exit repair kernel and the pointwise kernels of one outer iteration.
+// See gap_fused.hip for the algorithm.
+#pragma once
+
+#include "tv_tile.h"
+
+namespace scigap {
+namespace fused {
+
+
+// TV_ONLY runs the stencil alone (denoise_tv_chambolle): img = x_in, no
+// projection, no residual epilogue.
+enum Mode { GAP_ACC = 0, GAP_PLAIN = 1, ADMM = 2, TV_ONLY = 3 };
+
+struct FusedArgs {
+  const float* x_in;    // X_k (T,H,W); unused when from_y (X_0 = At(y))
+  float* x_out;         // X_{k+1}
+  const float* phi;     // (T,H,W)
+  const float* y;       // (H,W)
+  const float* phi_sum; // (H,W)
+  const float* r_in;    // residual of this iteration
+  float* r_out;         // residual of the next iteration
+  const float* y1_in;   // accelerated GAP: y1 of this iteration (after its update)
+  float* y1_out;
+  const float* truth;   // optional (T,H,W)
+  double* err;          // sum of squared errors of this iteration
+  float* b;             // ADMM b (T,H,W)
+  double* energy;       // [T][D] energies of this iteration
+  int* stats;           // frames that stopped early (FIX adds)
+  int H, W, T;
+  float lam, gam_next, weight, tw;
+  double eps;
+  int mode, clip, from_y, write_b, read_b, epilogue;
+  int tiles_x, ntiles, per_xcd;
+};
+
+// XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so
+// give XCD i a contiguous band of tiles and let neighbouring tiles (which
+// re-read each other's halos) share that XCD's L2.
+__device__ __forceinline__ int tile_of_block(int bid, int per_xcd) {
+  return (bid & 7) * per_xcd + (bid >> 3);
+}
+
+// Load an RY x 2 register block of one plane (zero outside the image).
+// `off` is the plane offset of register (0,0).  INSIDE tiles lie entirely in
+// the image (and W is even), so they use unguarded 8-byte loads with unsigned
+// offsets (scalar base + 32-bit vector offset addressing).
+template <int RY, bool INSIDE>
+__device__ __forceinline__ void load_plane(const float* __restrict__ src, float (&v)[RY][2],
+                                           const LanePos& lp, int off, int H, int W) {
+  if constexpr (INSIDE) {
+#pragma unroll
+    for (int j = 0; j < RY; ++j) {
+      float2 q = *reinterpret_cast<const float2*>(src + unsigned(off + j * W));
+      v[j][0] = q.x;
+      v[j][1] = q.y;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < RY; ++j) {
+      int r = lp.row0 + j;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        int c = lp.col + k;
+        v[j][k] = (r >= 0 && r < H && c >= 0 && c < W) ? src[unsigned(off + j * W + k)] : 0.f;
+      }
+    }
+  }
+}
+
+// Tile placement shared by the main and repair passes.
+template <int D, int BY, int MR>
+struct TilePlace {
+  using G = TileGeom<D, BY, MR>;
+  LanePos lp;
+  int off0;        // plane offset of the lane's register (0,0)
+  bool inside;     // load tile entirely inside the image and W even
+  bool lane_owner; // lane's columns are interior columns of the tile
+  bool own_c[2];   // interior column inside the image
+  unsigned own_r;  // bit j: row0+j is an interior row inside the image
+  __device__ TilePlace(int tile, int tiles_x, int H, int W) {
+    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+    const int c0 = tx * G::OX - G::HC, r0 = ty * G::OY - D;
+    lp.w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    lp.lane = threadIdx.x & 63;
+    lp.row0 = r0 + lp.w * G::RY;
+    lp.col = c0 + 2 * lp.lane;
+    off0 = lp.row0 * W + lp.col;
+    inside = c0 >= 0 && r0 >= 0 && c0 + G::LW <= W && r0 + G::LH <= H && (W % 2 == 0);
+    lane_owner = lp.lane >= G::HC / 2 && lp.lane < 64 - G::HC / 2;
+    own_c[0] = lane_owner && lp.col < W;
+    own_c[1] = lane_owner && lp.col + 1 < W;
+    own_r = 0;
+#pragma unroll
+    for (int j = 0; j < G::RY; ++j) {
+      const int tr = lp.w * G::RY + j;  // row within the load tile
+      own_r |= unsigned(tr >= D && tr < G::LH - D && lp.row0 + j < H) << j;
+    }
+  }
+  __device__ bool owns(int j, int k) const { return own_c[k] && ((own_r >> j) & 1u); }
+};
+
+// Residual of the next iteration at one owned pixel.
+template <int MODE>
+__device__ __forceinline__ void epilogue_pixel(const FusedArgs& a, unsigned o, float yb) {
+  float yv = a.y[o], s = a.phi_sum[o];
+  if constexpr (MODE == GAP_ACC) {
+    float y1 = a.y1_in[o] + (yv - yb);
+    a.y1_out[o] = y1;
+    a.r_out[o] = (y1 - yb) / s;
+  } else if constexpr (MODE == GAP_PLAIN) {
+    a.r_out[o] = (yv - yb) / s;
+  } else {
+    a.r_out[o] = (yv - yb) / (s + a.gam_next);
+  }
+}
+
+template <int BY>
+__device__ __forceinline__ void commit_err(float err, double* dst, double* red64, int w, int lane) {
+  double e = wave_sum(double(err));
+  if (lane == 0) red64[w] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int q = 0; q < BY; ++q) tot += red64[q];
+    atomicAdd(dst, tot);
+  }
+}
+
+// Main pass.  INSIDE selects the interior tiles (no masks, no bounds checks)
+// or the border ring; IQA adds the squared error against the truth.
+// Registers hold the stencil state (img, out, p0, p1) and the prefetched next
+// frame (x, Phi); lambda*r, the current Phi and the yb accumulator, each used
+// once per frame, live in LDS as [tile row][128] float planes.
+// LDS of one fused workgroup.
+template <int D, int BY, int MR>
+struct FusedLds {
+  static constexpr int LH = TileGeom<D, BY, MR>::LH;
+  TileLds<BY> eng;
+  float2 q[LH * 64], ph[LH * 64], yb[LH * 64], img[LH * 64];
+  float red[BY * D];
+  double red64[BY];
+};
+
+// One tile of one fused outer iteration (the body of k_fused and of the
+// batched persistent kernel).  Returns without work when the tile's class
+// (interior/border) differs from INSIDE.
+template <int D, int BY, int MR, int MODE, bool INSIDE, bool IQA>
+__device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
+                                           FusedLds<D, BY, MR>& sh) {
+  using G = TileGeom<D, BY, MR>;
+  constexpr int RY = G::RY;
+  constexpr int NH = MR;
+  TileLds<BY>& lds = sh.eng;
+  float2* const s_q = sh.q;
+  float2* const s_ph = sh.ph;
+  float2* const s_yb = sh.yb;
+  float2* const s_img = sh.img;
+  float* const red = sh.red;
+  double* const red64 = sh.red64;
+  const int H = a.H, W = a.W, T = a.T;
+  const int N = H * W;
+  const TilePlace<D, BY, MR> tp(tile, a.tiles_x, H, W);
+  if (tp.inside != INSIDE) return;
+  const LanePos& lp = tp.lp;
+  const int off0 = tp.off0;
+  // LDS slot of register row j of this lane: (w*RY + j)*64 + lane.
+  const int sl0 = lp.w * RY * 64 + lp.lane;
+
+  EdgeMasks em;
+  if constexpr (!INSIDE) em.init<RY>(lp, H, W);
+
+  const bool from_y = a.from_y;
+  if constexpr (MODE != TV_ONLY) {
+    // q = lambda*r (or y + lambda*r when X_0 = At(y)) over the load tile.
+    float q[RY][2];
+    load_plane<RY, INSIDE>(a.r_in, q, lp, off0, H, W);
+    if (from_y) {
+      float yv[RY][2];
+      load_plane<RY, INSIDE>(a.y, yv, lp, off0, H, W);
+#pragma unroll
+      for (int j = 0; j < RY; ++j)
+        s_q[sl0 + j * 64] = make_float2(fmaf(a.lam, q[j][0], yv[j][0]), fmaf(a.lam, q[j][1], yv[j][1]));
+    } else {
+#pragma unroll
+      for (int j = 0; j < RY; ++j) s_q[sl0 + j * 64] = make_float2(a.lam * q[j][0], a.lam * q[j][1]);
+    }
+#pragma unroll
+    for (int j = 0; j < RY; ++j) s_yb[sl0 + j * 64] = make_float2(0.f, 0.f);
+  }
+  float err = 0.f;
+
+  // Frame t+1's x and Phi are requested before frame t's TV so that their
+  // latency hides behind the stencil arithmetic.
+  float nx[RY][2], nph[RY][2];
+  const float* phit = a.phi;
+  const float* xt = a.x_in;
+  if constexpr (MODE != TV_ONLY) load_plane<RY, INSIDE>(phit, nph, lp, off0, H, W);
+  if (!from_y) load_plane<RY, INSIDE>(xt, nx, lp, off0, H, W);
+
+  float* xo = a.x_out;
+  const float* trt = a.truth;
+  float* bt = a.b;
+  double* en = a.energy;
+  // Every lane reads back only the LDS slots it wrote itself, so the planes
+  // s_q, s_ph, s_img and s_yb need no barriers.
+  float2* const img = s_img + sl0;
+  for (int t = 0; t < T; ++t) {
+#pragma unroll
+    for (int j = 0; j < RY; ++j) {
+      if constexpr (MODE == TV_ONLY) {
+        img[j * 64] = make_float2(nx[j][0], nx[j][1]);
+      } else {
+        const float2 qj = s_q[sl0 + j * 64];
+        s_ph[sl0 + j * 64] = make_float2(nph[j][0], nph[j][1]);
+        img[j * 64] = make_float2(from_y ? qj.x * nph[j][0] : fmaf(qj.x, nph[j][0], nx[j][0]),
+                                  from_y ? qj.y * nph[j][1] : fmaf(qj.y, nph[j][1], nx[j][1]));
+      }
+    }
+    if (t + 1 < T) {
+      if constexpr (MODE != TV_ONLY) load_plane<RY, INSIDE>(phit + N, nph, lp, off0, H, W);
+      if (!from_y) load_plane<RY, INSIDE>(xt + N, nx, lp, off0, H, W);
+    }
+
+    float out[RY][2];
+    float acc[NH * 8];
+    tv_run_from_zero<D, RY, BY, NH, !INSIDE>(img, out, D, lds, lp, em, a.weight, a.tw, acc, true);
+
+#pragma unroll
+    for (int j = 0; j < RY; ++j) {
+      if (!((tp.own_r >> j) & 1u)) continue;
+      if constexpr (MODE == TV_ONLY) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+          if (tp.own_c[k]) xo[unsigned(off0 + j * W + k)] = out[j][k];
+        continue;
+      }
+      const float2 ph = s_ph[sl0 + j * 64];
+      const float2 im = img[j * 64];
+      float2 yb = s_yb[sl0 + j * 64];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (!tp.own_c[k]) continue;
+        const unsigned o = unsigned(off0 + j * W + k);
+        const float f = k ? ph.y : ph.x;
+        const float v = k ? im.y : im.x;
+        float& ybk = k ? yb.y : yb.x;
+        float xn = out[j][k];
+        if constexpr (MODE == ADMM) {
+          if (a.clip) xn = fminf(fmaxf(xn, 0.f), 1.f);
+          if constexpr (IQA) {  // x_k = v_k + b_k
+            float bk = a.read_b ? bt[o] : 0.f;
+            float e = trt[o] - (v + bk);
+            err = fmaf(e, e, err);
+          }
+          if (a.write_b) bt[o] = xn - v;
+          ybk = fmaf(f, 2.f * xn - v, ybk);
+        } else {
+          if constexpr (IQA) {
+            float e = trt[o] - xn;
+            err = fmaf(e, e, err);
+          }
+          ybk = fmaf(f, xn, ybk);
+        }
+        xo[o] = xn;
+      }
+      s_yb[sl0 + j * 64] = yb;
+    }
+    energy_commit<D, BY, MR, NH>(acc, tp.lane_owner, en, red, lp);
+    phit += N;
+    xt += N;
+    xo += N;
+    en += D;
+    if constexpr (IQA) trt += N;
+    if constexpr (MODE == ADMM) bt += (bt ? N : 0);
+  }
+
+  if (MODE != TV_ONLY && a.epilogue) {
+#pragma unroll
+    for (int j = 0; j < RY; ++j) {
+      if (!((tp.own_r >> j) & 1u)) continue;
+      const float2 yb = s_yb[sl0 + j * 64];
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+        if (tp.own_c[k]) epilogue_pixel<MODE>(a, unsigned(off0 + j * W + k), k ? yb.y : yb.x);
+    }
+  }
+  if constexpr (IQA) commit_err<BY>(err, a.err, red64, lp.w, lp.lane);
+}
+
+template <int D, int BY, int MR, int MODE, bool INSIDE, bool IQA>
+__global__ void __launch_bounds__(64 * BY) k_fused(FusedArgs a) {
+  __shared__ FusedLds<D, BY, MR> sh;
+  const int tile = tile_of_block(blockIdx.x, a.per_xcd);
+  if (tile >= a.ntiles) return;
+  fused_tile<D, BY, MR, MODE, INSIDE, IQA>(a, tile, sh);
+}
+
+// Batched form: a persistent grid pulls (solve, tile) entries from a queue
+// with an atomic counter until it is drained.  Each solve has its own
+// FusedArgs (shape, buffers, parameters); the queue is ordered by the host
+// (largest work first) so that the long tiles start early.
+template <int D, int BY, int MR, int MODE, bool INSIDE>
+__global__ void __launch_bounds__(64 * BY) k_fused_batch(const FusedArgs* __restrict__ descs,
+                                                        const int2* __restrict__ queue, int nq,
+                                                        int* counter) {
+  __shared__ FusedLds<D, BY, MR> sh;
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int q = s_item;
+    __syncthreads();
+    if (q >= nq) break;
+    const int2 it = queue[q];
+    fused_tile<D, BY, MR, MODE, INSIDE, false>(descs[it.x], it.y, sh);
+    __syncthreads();
+  }
+}
+
+// Early-exit repair pass (rarely does work).  Every workgroup first derives
+// the exit iteration of every frame from the energies of the main pass; if no
+// frame stopped early it returns.  Otherwise it recomputes the frames that
+// stopped at s < D from the unchanged inputs, rewrites them, and recomputes
+// the epilogue of its tile from all frames.
+template <int D, int BY, int MR>
+__global__ void __launch_bounds__(64 * BY) k_fused_fix(FusedArgs a) {
+  using G = TileGeom<D, BY, MR>;
+  constexpr int RY = G::RY;
+  constexpr int NH = MR;
+  __shared__ TileLds<BY> lds;
+  __shared__ float2 s_img[G::LH * 64];
+  __shared__ double red64[BY];
+
+  const int tile = tile_of_block(blockIdx.x, a.per_xcd);
+  if (tile >= a.ntiles) return;
+  const int H = a.H, W = a.W, T = a.T;
+  const int N = H * W;
+  int nexit = 0;
+  for (int t = 0; t < T; ++t) nexit += exit_iteration<D>(a.energy + size_t(t) * D, a.eps) < D;
+  if (nexit == 0) return;
+  if (tile == 0 && threadIdx.x == 0 && a.stats) atomicAdd(a.stats, nexit);
+
+  const TilePlace<D, BY, MR> tp(tile, a.tiles_x, H, W);
+  const LanePos& lp = tp.lp;
+  const int off0 = tp.off0;
+  EdgeMasks em;
+  em.init<RY>(lp, H, W);
+
+  const bool tv_only = a.mode == TV_ONLY;
+  float q[RY][2];
+  if (tv_only) {
+#pragma unroll
+    for (int j = 0; j < RY; ++j) q[j][0] = q[j][1] = 0.f;
+  } else {
+    load_plane<RY, false>(a.r_in, q, lp, off0, H, W);
+  }
+  if (a.from_y) {
+    float yv[RY][2];
+    load_plane<RY, false>(a.y, yv, lp, off0, H, W);
+#pragma unroll
+    for (int j = 0; j < RY; ++j)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) q[j][k] = fmaf(a.lam, q[j][k], yv[j][k]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < RY; ++j)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) q[j][k] *= a.lam;
+  }
+  float yb[RY][2];
+#pragma unroll
+  for (int j = 0; j < RY; ++j) yb[j][0] = yb[j][1] = 0.f;
+  float err = 0.f;
+
+  for (int t = 0; t < T; ++t) {
+    const float* phit = a.phi + size_t(t) * N;
+    const float* xt = a.x_in + size_t(t) * N;
+    float* xo = a.x_out + size_t(t) * N;
+    const int nup = exit_iteration<D>(a.energy + size_t(t) * D, a.eps);
+    if (nup == D && tv_only) continue;
+    if (nup == D) {
+      // Frame was right: only its contribution to the epilogue is needed.
+#pragma unroll
+      for (int j = 0; j < RY; ++j)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          if (!tp.owns(j, k)) continue;
+          const unsigned o = unsigned(off0 + j * W + k);
+          float f = phit[o], xn = xo[o];
+          if (a.mode == ADMM) {
+            float v = a.from_y ? q[j][k] * f : fmaf(q[j][k], f, xt[o]);
+            yb[j][k] = fmaf(f, 2.f * xn - v, yb[j][k]);
+          } else {
+            yb[j][k] = fmaf(f, xn, yb[j][k]);
+          }
+        }
+      continue;
+    }
+    float ph[RY][2], img[RY][2], xv[RY][2];
+    if (tv_only) {
+#pragma unroll
+      for (int j = 0; j < RY; ++j) ph[j][0] = ph[j][1] = 0.f;
+    } else {
+      load_plane<RY, false>(phit, ph, lp, off0, H, W);
+    }
+    if (!a.from_y) load_plane<RY, false>(xt, xv, lp, off0, H, W);
+    float2* const simg = s_img + lp.w * RY * 64 + lp.lane;
+#pragma unroll
+    for (int j = 0; j < RY; ++j) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+        img[j][k] = a.from_y ? q[j][k] * ph[j][k] : fmaf(q[j][k], ph[j][k], xv[j][k]);
+      simg[j * 64] = make_float2(img[j][0], img[j][1]);
+    }
+    float out[RY][2];
+    float acc[NH * 8];
+    tv_run_from_zero<D, RY, BY, NH, true>(simg, out, nup, lds, lp, em, a.weight, a.tw, acc, false);
+    const float* trt = a.truth ? a.truth + size_t(t) * N : nullptr;
+    float* bt = a.b ? a.b + size_t(t) * N : nullptr;
+#pragma unroll
+    for (int j = 0; j < RY; ++j)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (!tp.owns(j, k)) continue;
+        const unsigned o = unsigned(off0 + j * W + k);
+        float xn = out[j][k];
+        if (tv_only) {
+          xo[o] = xn;
+          continue;
+        }
+        if (a.mode == ADMM) {
+          if (a.clip) xn = fminf(fmaxf(xn, 0.f), 1.f);
+          if (a.write_b) bt[o] = xn - img[j][k];
+          yb[j][k] = fmaf(ph[j][k], 2.f * xn - img[j][k], yb[j][k]);
+        } else {
+          if (trt) {  // replace the main pass's error of this pixel
+            float e = trt[o] - xn, eo = trt[o] - xo[o];
+            err = fmaf(e, e, err);
+            err = fmaf(-eo, eo, err);
+          }
+          yb[j][k] = fmaf(ph[j][k], xn, yb[j][k]);
+        }
+        xo[o] = xn;
+      }
+  }
+  if (!tv_only && a.epilogue) {
+#pragma unroll
+    for (int j = 0; j < RY; ++j)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (!tp.owns(j, k)) continue;
+        const unsigned o = unsigned(off0 + j * W + k);
+        if (a.mode == GAP_ACC) epilogue_pixel<GAP_ACC>(a, o, yb[j][k]);
+        else if (a.mode == GAP_PLAIN) epilogue_pixel<GAP_PLAIN>(a, o, yb[j][k]);
+        else epilogue_pixel<ADMM>(a, o, yb[j][k]);
+      }
+  }
+  if (a.truth && a.err && a.mode != ADMM && !tv_only) commit_err<BY>(err, a.err, red64, lp.w, lp.lane);
+}
+
+// Initial residual: X_0 = x0 (or At(y)), yb = A(X_0), then
+// GAP-acc: y1 = y - yb, r = (y1 - yb)/Phi_sum; GAP-plain: r = (y - yb)/Phi_sum;
+// ADMM: r = (y - yb)/(Phi_sum + gamma)   (pnp_sci_algo.py:625-633, 640-645, 808-809)
+__global__ void __launch_bounds__(256) k_init(const float* __restrict__ x0,
+                                              const float* __restrict__ phi,
+                                              const float* __restrict__ y,
+                                              const float* __restrict__ phi_sum,
+                                              float* __restrict__ r, float* __restrict__ y1,
+                                              int64_t N, int T, int mode, float gam) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    float yv = y[i], yb = 0.f;
+    for (int t = 0; t < T; ++t) {
+      float f = phi[t * N + i];
+      float xv = x0 ? x0[t * N + i] : yv * f;
+      yb = fmaf(xv, f, yb);
+    }
+    float s = phi_sum[i];
+    if (mode == GAP_ACC) {
+      float v1 = yv - yb;
+      y1[i] = v1;
+      r[i] = (v1 - yb) / s;
+    } else if (mode == GAP_PLAIN) {
+      r[i] = (yv - yb) / s;
+    } else {
+      r[i] = (yv - yb) / (s + gam);
+    }
+  }
+}
+
+// Last ADMM iteration: x = theta + b + lambda*r*Phi (pnp_sci_algo.py:809), in
+// place on x_io when theta lives there, and its squared error.
+__global__ void __launch_bounds__(256) k_admm_final(const float* __restrict__ theta,
+                                                    const float* __restrict__ b,
+                                                    const float* __restrict__ r,
+                                                    const float* __restrict__ phi,
+                                                    const float* __restrict__ y, float* x_out,
+                                                    const float* __restrict__ truth, double* err,
+                                                    int64_t N, int T, float lam, int from_y) {
+  float e = 0.f;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    float lr = lam * r[i];
+    float yv = from_y ? y[i] : 0.f;
+    for (int t = 0; t < T; ++t) {
+      size_t o = size_t(t) * N + i;
+      float f = phi[o];
+      float th = from_y ? yv * f : theta[o];
+      float x = fmaf(lr, f, th + (b ? b[o] : 0.f));
+      x_out[o] = x;
+      if (truth) {
+        float d = truth[o] - x;
+        e = fmaf(d, d, e);
+      }
+    }
+  }
+  if (truth && err) {
+    double s = wave_sum(double(e));
+    if ((threadIdx.x & 63) == 0) atomicAdd(err, s);
+  }
+}
+
+// Outer iteration with an identity denoiser (tv_iter_max = 1: scikit-image
+// returns its input unchanged): x' = x + lambda*r*Phi per pixel, then the
+// next residual.  Pointwise, so x may be updated in place.
+__global__ void __launch_bounds__(256) k_project(FusedArgs a) {
+  const int64_t N = int64_t(a.H) * a.W;
+  float err = 0.f;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const float lr = a.lam * a.r_in[i];
+    const float yv = a.y[i];
+    float yb = 0.f;
+    for (int t = 0; t < a.T; ++t) {
+      const size_t o = size_t(t) * N + i;
+      const float f = a.phi[o];
+      const float xk = a.from_y ? yv * f : a.x_in[o];
+      const float v = fmaf(lr, f, xk);
+      float xn = v;
+      if (a.mode == ADMM) {
+        if (a.clip) xn = fminf(fmaxf(xn, 0.f), 1.f);
+        if (a.truth) {
+          float e = a.truth[o] - (v + (a.read_b ? a.b[o] : 0.f));
+          err = fmaf(e, e, err);
+        }
+        if (a.write_b) a.b[o] = xn - v;
+        yb = fmaf(f, 2.f * xn - v, yb);
+      } else {
+        if (a.truth) {
+          float e = a.truth[o] - xn;
+          err = fmaf(e, e, err);
+        }
+        yb = fmaf(f, xn, yb);
+      }
+      a.x_out[o] = xn;
+    }
+    if (a.epilogue) {
+      if (a.mode == GAP_ACC) epilogue_pixel<GAP_ACC>(a, unsigned(i), yb);
+      else if (a.mode == GAP_PLAIN) epilogue_pixel<GAP_PLAIN>(a, unsigned(i), yb);
+      else epilogue_pixel<ADMM>(a, unsigned(i), yb);
+    }
+  }
+  if (a.truth && a.err) {
+    double s = wave_sum(double(err));
+    if ((threadIdx.x & 63) == 0) atomicAdd(a.err, s);
+  }
+}
+
+// Deep-TV path (tv_iter_max-1 beyond the fused depth), step 1: the TV input
+// v = x + lambda*r*Phi (GAP) or theta + lambda*r*Phi (ADMM), written to x_out.
+__global__ void __launch_bounds__(256) k_gap_proj(FusedArgs a) {
+  const int64_t N = int64_t(a.H) * a.W;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const float lr = a.lam * a.r_in[i];
+    const float yv = a.from_y ? a.y[i] : 0.f;
+    for (int t = 0; t < a.T; ++t) {
+      const size_t o = size_t(t) * N + i;
+      const float f = a.phi[o];
+      a.x_out[o] = fmaf(lr, f, a.from_y ? yv * f : a.x_in[o]);
+    }
+  }
+}
+
+// Deep-TV path, step 3: with x_out = TV(v) and v in `vin` (passed as x_in),
+// clip/b/error bookkeeping and the next residual, as in the fused epilogue.
+__global__ void __launch_bounds__(256) k_gap_epi(FusedArgs a) {
+  const int64_t N = int64_t(a.H) * a.W;
+  const float* vin = a.x_in;
+  float err = 0.f;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    float yb = 0.f;
+    for (int t = 0; t < a.T; ++t) {
+      const size_t o = size_t(t) * N + i;
+      const float f = a.phi[o];
+      float xn = a.x_out[o];
+      if (a.mode == ADMM) {
+        const float v = vin[o];
+        if (a.clip) {
+          xn = fminf(fmaxf(xn, 0.f), 1.f);
+          a.x_out[o] = xn;
+        }
+        if (a.truth) {
+          float e = a.truth[o] - (v + (a.read_b ? a.b[o] : 0.f));
+          err = fmaf(e, e, err);
+        }
+        if (a.write_b) a.b[o] = xn - v;
+        yb = fmaf(f, 2.f * xn - v, yb);
+      } else {
+        if (a.truth) {
+          float e = a.truth[o] - xn;
+          err = fmaf(e, e, err);
+        }
+        yb = fmaf(f, xn, yb);
+      }
+    }
+    if (a.epilogue) {
+      if (a.mode == GAP_ACC) epilogue_pixel<GAP_ACC>(a, unsigned(i), yb);
+      else if (a.mode == GAP_PLAIN) epilogue_pixel<GAP_PLAIN>(a, unsigned(i), yb);
+      else epilogue_pixel<ADMM>(a, unsigned(i), yb);
+    }
+  }
+  if (a.truth && a.err) {
+    double s = wave_sum(double(err));
+    if ((threadIdx.x & 63) == 0) atomicAdd(a.err, s);
+  }
+}
+
+__global__ void k_fill_i32(int* p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+
+}  // namespace fused
+}  // namespace scigap

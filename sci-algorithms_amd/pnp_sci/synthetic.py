@@ -30,10 +30,20 @@ __all__ = ["Problem", "truth_video", "make_problem", "config1", "config2", "conf
 
 @dataclass
 class Problem:
-    truth: np.ndarray    # (H,W,T) float32
-    mask: np.ndarray     # (H,W,T) float32
-    y: np.ndarray        # (H,W) float32
-    phi_sum: np.ndarray  # (H,W) float32, zeros replaced by 1
+    truth_thw: np.ndarray  # (T,H,W) float32, contiguous
+    mask_thw: np.ndarray   # (T,H,W) float32, contiguous
+    y: np.ndarray          # (H,W) float32
+    phi_sum: np.ndarray    # (H,W) float32, zeros replaced by 1
+
+    @property
+    def truth(self) -> np.ndarray:
+        """(H,W,T) view in the reference's layout."""
+        return self.truth_thw.transpose(1, 2, 0)
+
+    @property
+    def mask(self) -> np.ndarray:
+        """(H,W,T) view in the reference's layout."""
+        return self.mask_thw.transpose(1, 2, 0)
 
 
 def _lowpass(noise: np.ndarray, sigma: float) -> np.ndarray:
@@ -44,7 +54,9 @@ def _lowpass(noise: np.ndarray, sigma: float) -> np.ndarray:
     return np.fft.irfft2(np.fft.rfft2(noise) * g, s=(H, W))
 
 
-def truth_video(rng: np.random.Generator, H: int, W: int, T: int, nblobs: int = 5) -> np.ndarray:
+def truth_video(rng: np.random.Generator, H: int, W: int, T: int, nblobs: int = 5,
+                frame_major: bool = False) -> np.ndarray:
+    """(H,W,T) video (or (T,H,W) with frame_major) in [0,1], float32."""
     bg = _lowpass(rng.random((H, W)), min(H, W) / 32.0)
     lo, hi = bg.min(), bg.max()
     bg = 0.5 * (bg - lo) / (hi - lo if hi > lo else 1.0)
@@ -59,7 +71,7 @@ def truth_video(rng: np.random.Generator, H: int, W: int, T: int, nblobs: int = 
         params.append((sig, amp, r0, c0, head, speed))
     rows = np.arange(H, dtype=np.float64)
     cols = np.arange(W, dtype=np.float64)
-    video = np.empty((H, W, T), dtype=np.float32)
+    video = np.empty((T, H, W), dtype=np.float32)
     for t in range(T):
         frame = bg.copy()
         for sig, amp, r0, c0, head, speed in params:
@@ -67,29 +79,31 @@ def truth_video(rng: np.random.Generator, H: int, W: int, T: int, nblobs: int = 
             cc = c0 + t * speed * np.cos(head)
             gr = np.exp(-((rows - rc) ** 2) / (2 * sig * sig))
             gc = np.exp(-((cols - cc) ** 2) / (2 * sig * sig))
-            frame += amp * np.outer(gr, gc)
-        video[:, :, t] = np.clip(frame, 0.0, 1.0)
-    return video
+            frame += np.outer(amp * gr, gc)
+        np.clip(frame, 0.0, 1.0, out=frame)
+        video[t] = frame
+    return video if frame_major else video.transpose(1, 2, 0)
 
 
 def make_problem(H: int, W: int, T: int, *, seed: int = 0, mask: str = "bernoulli",
                  p: float = 0.5, noise_rel: float = 0.0, rng=None) -> Problem:
     rng = rng if rng is not None else np.random.default_rng(seed)
-    truth = truth_video(rng, H, W, T)
+    truth = truth_video(rng, H, W, T, frame_major=True)
     if mask == "bernoulli":
-        phi = (rng.random((H, W, T)) < p).astype(np.float32)
+        # drawn in (H,W,T) order so the mask of a pixel's frames is contiguous in the stream
+        phi = np.ascontiguousarray((rng.random((H, W, T)) < p).astype(np.float32).transpose(2, 0, 1))
     elif mask == "shift":
         base = (rng.random((H, W)) < p).astype(np.float32)
-        phi = np.stack([np.roll(base, t, axis=0) for t in range(T)], axis=-1)
+        phi = np.stack([np.roll(base, t, axis=0) for t in range(T)], axis=0)
     else:
         raise ValueError("unknown mask kind %r" % mask)
-    y = (truth.astype(np.float64) * phi).sum(axis=2)
+    y = np.einsum("thw,thw->hw", truth, phi, dtype=np.float64)
     if noise_rel > 0:
         y = y + rng.normal(0.0, noise_rel * y.max(), size=y.shape)
     y = y.astype(np.float32)
-    ps = phi.sum(axis=2)
+    ps = phi.sum(axis=0)
     ps[ps == 0] = 1
-    return Problem(truth=truth, mask=phi, y=y, phi_sum=ps.astype(np.float32))
+    return Problem(truth_thw=truth, mask_thw=phi, y=y, phi_sum=ps.astype(np.float32))
 
 
 def config1(seed: int = 0) -> Problem:
