@@ -28,7 +28,10 @@ struct FusedArgs {
   double* err;          // sum of squared errors of this iteration
   float* b;             // ADMM b (T,H,W)
   double* energy;       // [T][D] energies of this iteration
-  int* stats;           // frames that stopped early (FIX adds)
+  float* yb;            // (H,W) sum_t Phi_t*x_t^{new} on owned pixels (for the repair)
+  int* stop;            // [T] exit iteration per frame (written by k_decide)
+  int* nexit;           // frames of this iteration that stopped early (k_decide)
+  int* stats;           // optional copy of *nexit for the caller
   int H, W, T;
   float lam, gam_next, weight, tw;
   double eps;
@@ -43,28 +46,50 @@ __device__ __forceinline__ int tile_of_block(int bid, int per_xcd) {
   return (bid & 7) * per_xcd + (bid >> 3);
 }
 
+// Buffer-resource access to one (H,W) plane.  A raw buffer access addresses
+// base + 32-bit lane offset + uniform (scalar) offset and returns 0 (or drops
+// the store) outside [0, N*4) bytes, so a plane needs one VGPR of address per
+// lane and an out-of-range offset can never fault.  Planes are < 4 GiB
+// (H*W < 2^30, checked by the host).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float* p, int N) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, unsigned(N) * 4u,
+                                           0x00020000);
+}
+__device__ __forceinline__ float ld1(const float* p, int N, unsigned o) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(plane_rsrc(p, N), o * 4u,
+                                                                        0, 0));
+}
+__device__ __forceinline__ void st1(float* p, int N, unsigned o, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), plane_rsrc(p, N), o * 4u, 0, 0);
+}
+
 // Load an RY x 2 register block of one plane (zero outside the image).
 // `off` is the plane offset of register (0,0).  INSIDE tiles lie entirely in
-// the image (and W is even), so they use unguarded 8-byte loads with unsigned
-// offsets (scalar base + 32-bit vector offset addressing).
+// the image (and W is even): one 8-byte buffer load per row, the row step in
+// the scalar offset.  Border tiles send pixels outside the image to an offset
+// past the end of the buffer, which reads as 0.
 template <int RY, bool INSIDE>
 __device__ __forceinline__ void load_plane(const float* __restrict__ src, float (&v)[RY][2],
                                            const LanePos& lp, int off, int H, int W) {
+  const __amdgpu_buffer_rsrc_t rs = plane_rsrc(src, H * W);
   if constexpr (INSIDE) {
+    const unsigned vo = unsigned(off) * 4u;
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
-      float2 q = *reinterpret_cast<const float2*>(src + unsigned(off + j * W));
-      v[j][0] = q.x;
-      v[j][1] = q.y;
+      const auto q = __builtin_amdgcn_raw_buffer_load_b64(rs, vo, j * W * 4, 0);
+      v[j][0] = __builtin_bit_cast(float, q[0]);
+      v[j][1] = __builtin_bit_cast(float, q[1]);
     }
   } else {
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
-      int r = lp.row0 + j;
+      const int r = lp.row0 + j;
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
-        int c = lp.col + k;
-        v[j][k] = (r >= 0 && r < H && c >= 0 && c < W) ? src[unsigned(off + j * W + k)] : 0.f;
+        const int c = lp.col + k;
+        const bool in = r >= 0 && r < H && c >= 0 && c < W;
+        const unsigned o = in ? unsigned(off + j * W + k) * 4u : 0xFFFFFFF0u;
+        v[j][k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, o, 0, 0));
       }
     }
   }
@@ -234,7 +259,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
       if constexpr (MODE == TV_ONLY) {
 #pragma unroll
         for (int k = 0; k < 2; ++k)
-          if (tp.own_c[k]) xo[unsigned(off0 + j * W + k)] = out[j][k];
+          if (tp.own_c[k]) st1(xo, N, unsigned(off0 + j * W + k), out[j][k]);
         continue;
       }
       const float2 ph = s_ph[sl0 + j * 64];
@@ -251,20 +276,20 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
         if constexpr (MODE == ADMM) {
           if (a.clip) xn = fminf(fmaxf(xn, 0.f), 1.f);
           if constexpr (IQA) {  // x_k = v_k + b_k
-            float bk = a.read_b ? bt[o] : 0.f;
-            float e = trt[o] - (v + bk);
+            float bk = a.read_b ? ld1(bt, N, o) : 0.f;
+            float e = ld1(trt, N, o) - (v + bk);
             err = fmaf(e, e, err);
           }
-          if (a.write_b) bt[o] = xn - v;
+          if (a.write_b) st1(bt, N, o, xn - v);
           ybk = fmaf(f, 2.f * xn - v, ybk);
         } else {
           if constexpr (IQA) {
-            float e = trt[o] - xn;
+            float e = ld1(trt, N, o) - xn;
             err = fmaf(e, e, err);
           }
           ybk = fmaf(f, xn, ybk);
         }
-        xo[o] = xn;
+        st1(xo, N, o, xn);
       }
       s_yb[sl0 + j * 64] = yb;
     }
@@ -277,14 +302,19 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
     if constexpr (MODE == ADMM) bt += (bt ? N : 0);
   }
 
-  if (MODE != TV_ONLY && a.epilogue) {
+  if (MODE != TV_ONLY) {
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
       if (!((tp.own_r >> j) & 1u)) continue;
       const float2 yb = s_yb[sl0 + j * 64];
 #pragma unroll
-      for (int k = 0; k < 2; ++k)
-        if (tp.own_c[k]) epilogue_pixel<MODE>(a, unsigned(off0 + j * W + k), k ? yb.y : yb.x);
+      for (int k = 0; k < 2; ++k) {
+        if (!tp.own_c[k]) continue;
+        const unsigned o = unsigned(off0 + j * W + k);
+        const float v = k ? yb.y : yb.x;
+        if (a.yb) st1(a.yb, N, o, v);
+        if (a.epilogue) epilogue_pixel<MODE>(a, o, v);
+      }
     }
   }
   if constexpr (IQA) commit_err<BY>(err, a.err, red64, lp.w, lp.lane);
@@ -320,11 +350,32 @@ __global__ void __launch_bounds__(64 * BY) k_fused_batch(const FusedArgs* __rest
   }
 }
 
-// Early-exit repair pass (rarely does work).  Every workgroup first derives
-// the exit iteration of every frame from the energies of the main pass; if no
-// frame stopped early it returns.  Otherwise it recomputes the frames that
-// stopped at s < D from the unchanged inputs, rewrites them, and recomputes
-// the epilogue of its tile from all frames.
+// Early-exit decision: one workgroup derives every frame's exit iteration
+// from the energies of the main pass (_denoise.py:384-391) and counts the
+// frames that stopped before D updates.
+template <int D>
+__global__ void __launch_bounds__(256) k_decide(const double* __restrict__ energy, int T,
+                                                double eps, int* stop, int* nexit, int* stats) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  for (int t = threadIdx.x; t < T; t += blockDim.x) {
+    const int s = exit_iteration<D>(energy + size_t(t) * D, eps);
+    stop[t] = s;
+    if (s < D) atomicAdd(&cnt, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *nexit = cnt;
+    if (stats) *stats = cnt;
+  }
+}
+
+// Early-exit repair (does work only when k_decide found an early stop).
+// For every frame that stopped at s < D, the frame is recomputed with s
+// updates from the unchanged inputs (x_in and r_in are double-buffered), its
+// contribution to yb is corrected by Phi*(x_new - x_old) (ADMM: 2*Phi*...),
+// and the residual epilogue is recomputed from the corrected yb.
 template <int D, int BY, int MR>
 __global__ void __launch_bounds__(64 * BY) k_fused_fix(FusedArgs a) {
   using G = TileGeom<D, BY, MR>;
@@ -334,22 +385,18 @@ __global__ void __launch_bounds__(64 * BY) k_fused_fix(FusedArgs a) {
   __shared__ float2 s_img[G::LH * 64];
   __shared__ double red64[BY];
 
+  if (*a.nexit == 0) return;  // the common case: one scalar load per workgroup
   const int tile = tile_of_block(blockIdx.x, a.per_xcd);
   if (tile >= a.ntiles) return;
   const int H = a.H, W = a.W, T = a.T;
   const int N = H * W;
-  int nexit = 0;
-  for (int t = 0; t < T; ++t) nexit += exit_iteration<D>(a.energy + size_t(t) * D, a.eps) < D;
-  if (nexit == 0) return;
-  if (tile == 0 && threadIdx.x == 0 && a.stats) atomicAdd(a.stats, nexit);
-
   const TilePlace<D, BY, MR> tp(tile, a.tiles_x, H, W);
   const LanePos& lp = tp.lp;
   const int off0 = tp.off0;
   EdgeMasks em;
   em.init<RY>(lp, H, W);
-
   const bool tv_only = a.mode == TV_ONLY;
+
   float q[RY][2];
   if (tv_only) {
 #pragma unroll
@@ -370,35 +417,17 @@ __global__ void __launch_bounds__(64 * BY) k_fused_fix(FusedArgs a) {
 #pragma unroll
       for (int k = 0; k < 2; ++k) q[j][k] *= a.lam;
   }
-  float yb[RY][2];
+  float dyb[RY][2];
 #pragma unroll
-  for (int j = 0; j < RY; ++j) yb[j][0] = yb[j][1] = 0.f;
+  for (int j = 0; j < RY; ++j) dyb[j][0] = dyb[j][1] = 0.f;
   float err = 0.f;
 
   for (int t = 0; t < T; ++t) {
+    const int nup = a.stop[t];
+    if (nup >= D) continue;
     const float* phit = a.phi + size_t(t) * N;
     const float* xt = a.x_in + size_t(t) * N;
     float* xo = a.x_out + size_t(t) * N;
-    const int nup = exit_iteration<D>(a.energy + size_t(t) * D, a.eps);
-    if (nup == D && tv_only) continue;
-    if (nup == D) {
-      // Frame was right: only its contribution to the epilogue is needed.
-#pragma unroll
-      for (int j = 0; j < RY; ++j)
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          if (!tp.owns(j, k)) continue;
-          const unsigned o = unsigned(off0 + j * W + k);
-          float f = phit[o], xn = xo[o];
-          if (a.mode == ADMM) {
-            float v = a.from_y ? q[j][k] * f : fmaf(q[j][k], f, xt[o]);
-            yb[j][k] = fmaf(f, 2.f * xn - v, yb[j][k]);
-          } else {
-            yb[j][k] = fmaf(f, xn, yb[j][k]);
-          }
-        }
-      continue;
-    }
     float ph[RY][2], img[RY][2], xv[RY][2];
     if (tv_only) {
 #pragma unroll
@@ -427,35 +456,37 @@ __global__ void __launch_bounds__(64 * BY) k_fused_fix(FusedArgs a) {
         if (!tp.owns(j, k)) continue;
         const unsigned o = unsigned(off0 + j * W + k);
         float xn = out[j][k];
-        if (tv_only) {
-          xo[o] = xn;
-          continue;
-        }
-        if (a.mode == ADMM) {
-          if (a.clip) xn = fminf(fmaxf(xn, 0.f), 1.f);
-          if (a.write_b) bt[o] = xn - img[j][k];
-          yb[j][k] = fmaf(ph[j][k], 2.f * xn - img[j][k], yb[j][k]);
-        } else {
-          if (trt) {  // replace the main pass's error of this pixel
-            float e = trt[o] - xn, eo = trt[o] - xo[o];
-            err = fmaf(e, e, err);
-            err = fmaf(-eo, eo, err);
+        const float xold = xo[o];
+        if (!tv_only) {
+          if (a.mode == ADMM) {
+            if (a.clip) xn = fminf(fmaxf(xn, 0.f), 1.f);
+            if (a.write_b) bt[o] = xn - img[j][k];
+            dyb[j][k] = fmaf(2.f * ph[j][k], xn - xold, dyb[j][k]);
+          } else {
+            if (trt) {  // replace the main pass's error of this pixel
+              float e = trt[o] - xn, eo = trt[o] - xold;
+              err = fmaf(e, e, err);
+              err = fmaf(-eo, eo, err);
+            }
+            dyb[j][k] = fmaf(ph[j][k], xn - xold, dyb[j][k]);
           }
-          yb[j][k] = fmaf(ph[j][k], xn, yb[j][k]);
         }
         xo[o] = xn;
       }
   }
-  if (!tv_only && a.epilogue) {
+  if (!tv_only) {
 #pragma unroll
     for (int j = 0; j < RY; ++j)
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         if (!tp.owns(j, k)) continue;
         const unsigned o = unsigned(off0 + j * W + k);
-        if (a.mode == GAP_ACC) epilogue_pixel<GAP_ACC>(a, o, yb[j][k]);
-        else if (a.mode == GAP_PLAIN) epilogue_pixel<GAP_PLAIN>(a, o, yb[j][k]);
-        else epilogue_pixel<ADMM>(a, o, yb[j][k]);
+        const float yb = a.yb[o] + dyb[j][k];
+        a.yb[o] = yb;
+        if (!a.epilogue) continue;
+        if (a.mode == GAP_ACC) epilogue_pixel<GAP_ACC>(a, o, yb);
+        else if (a.mode == GAP_PLAIN) epilogue_pixel<GAP_PLAIN>(a, o, yb);
+        else epilogue_pixel<ADMM>(a, o, yb);
       }
   }
   if (a.truth && a.err && a.mode != ADMM && !tv_only) commit_err<BY>(err, a.err, red64, lp.w, lp.lane);

@@ -38,7 +38,6 @@ int fill_f64(double* p, int n, double v, hipStream_t s);
 size_t tv_general_workspace(int64_t H, int64_t W, int64_t T, int n);
 int tv_general(const float* image, float* out, int64_t H, int64_t W, int64_t T, float weight,
                float eps, int n, int* stop_iter, void* ws, hipStream_t s);
-int tv_stop_from_energy(const double* energy, int T, int D, double eps, int* stop, hipStream_t s);
 
 namespace {
 
@@ -92,6 +91,11 @@ int launch_fused(const FusedArgs& a0, hipStream_t s) {
     rc = launch_main<D, TV_ONLY, false>(a, grid, s);
   if (rc) return rc;
   {
+    ProfScope ps("fused_decide", s);
+    k_decide<D><<<1, 256, 0, s>>>(a.energy, a.T, a.eps, a.stop, a.nexit, a.stats);
+  }
+  SCIGAP_LAUNCH_CHECK("early-exit decision launch");
+  {
     ProfScope ps("fused_fix", s);
     k_fused_fix<D, BY, MR><<<grid, 64 * BY, 0, s>>>(a);
   }
@@ -121,6 +125,9 @@ struct Workspace {
   float* bbuf = nullptr;  // T*N (ADMM)
   float* vbuf = nullptr;  // T*N TV input (deep-TV path)
   void* tvws = nullptr;   // general TV workspace (deep-TV path)
+  float* yb = nullptr;    // (H,W) yb of the latest fused launch
+  int* stop = nullptr;    // [T] exit iterations of the latest fused launch
+  int* nexit = nullptr;   // [iters] early-exit counts
   float* r[2] = {nullptr, nullptr};
   float* y1[2] = {nullptr, nullptr};
   double* energy = nullptr;  // iters * T * D
@@ -143,6 +150,9 @@ Workspace carve(const scigap_params* p, char* base) {
   if (p->method == SCIGAP_METHOD_GAP && p->accelerate)
     for (int i = 0; i < 2; ++i) w.y1[i] = reinterpret_cast<float*>(take(N * 4));
   w.energy = reinterpret_cast<double*>(take(size_t(p->iters > 0 ? p->iters : 1) * T * D * 8));
+  w.yb = reinterpret_cast<float*>(take(N * 4));
+  w.stop = reinterpret_cast<int*>(take(T * 4));
+  w.nexit = reinterpret_cast<int*>(take(size_t(p->iters > 0 ? p->iters : 1) * 4));
   if (p->tv_iter_max - 1 > kMaxFused) {
     w.vbuf = reinterpret_cast<float*>(take(T * N * 4));
     w.tvws = take(tv_general_workspace(p->H, p->W, p->T, p->tv_iter_max));
@@ -217,6 +227,8 @@ int scigap_solve(const scigap_params* p, const float* y, const float* phi, const
 
   const int D = p->tv_iter_max - 1;
   const bool deep = D > kMaxFused;
+  if (N >= (int64_t(1) << 30))
+    return fail(SCIGAP_EINVAL, "H*W must be < 2^30 (buffer-addressed planes)");
 
   hipStream_t s = as_stream(stream);
   const int K = p->iters;
@@ -280,6 +292,9 @@ int scigap_solve(const scigap_params* p, const float* y, const float* phi, const
     a.err = iqa ? sq_err + k : nullptr;
     a.b = ws.bbuf;
     a.energy = ws.energy + size_t(k) * T * D;
+    a.yb = ws.yb;
+    a.stop = ws.stop;
+    a.nexit = ws.nexit + k;
     a.stats = stats ? stats + k : nullptr;
     a.H = int(p->H);
     a.W = int(p->W);
@@ -331,7 +346,7 @@ size_t scigap_tv_workspace_bytes(int64_t H, int64_t W, int64_t T, int n_iter_max
   if (check_shape(H, W, T) != SCIGAP_OK || n_iter_max < 1) return 0;
   const int D = n_iter_max - 1;
   if (D == 0) return 256;
-  if (D <= kMaxFused) return align256(size_t(T) * D * 8);
+  if (D <= kMaxFused) return align256(size_t(T) * D * 8) + align256(size_t(T) * 4) + 256;
   return tv_general_workspace(H, W, T, n_iter_max);
 }
 
@@ -365,11 +380,14 @@ int scigap_tv_chambolle(const float* image, float* out, int64_t H, int64_t W, in
   if (D > kMaxFused)
     return tv_general(image, out, H, W, T, weight, eps, n_iter_max, stop_iter, workspace, s);
   double* energy = static_cast<double*>(workspace);
+  char* rest = static_cast<char*>(workspace) + align256(size_t(T) * D * 8);
   if (int rc = fill_f64(energy, int(T) * D, 0.0, s)) return rc;
   FusedArgs a{};
   a.x_in = image;
   a.x_out = out;
   a.energy = energy;
+  a.stop = stop_iter ? stop_iter : reinterpret_cast<int*>(rest);
+  a.nexit = reinterpret_cast<int*>(rest + align256(size_t(T) * 4));
   a.H = int(H);
   a.W = int(W);
   a.T = int(T);
@@ -377,9 +395,7 @@ int scigap_tv_chambolle(const float* image, float* out, int64_t H, int64_t W, in
   a.tw = float(0.25 / double(weight));
   a.eps = eps;
   a.mode = TV_ONLY;
-  if (int rc = launch_fused_depth(D, a, s)) return rc;
-  if (stop_iter) return tv_stop_from_energy(energy, int(T), D, eps, stop_iter, s);
-  return SCIGAP_OK;
+  return launch_fused_depth(D, a, s);
 }
 
 }  // extern "C"

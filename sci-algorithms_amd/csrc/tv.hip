@@ -150,11 +150,4 @@ int tv_general(const float* image, float* out, int64_t H, int64_t W, int64_t T, 
   return SCIGAP_OK;
 }
 
-// Stop iterations of the fused path, from its [T][D] energies.
-int tv_stop_from_energy(const double* energy, int T, int D, double eps, int* stop, hipStream_t s) {
-  k_stop_iter<<<unsigned((T + 255) / 256), 256, 0, s>>>(energy, T, D, D, D, eps, stop);
-  SCIGAP_LAUNCH_CHECK("tv stop launch");
-  return SCIGAP_OK;
-}
-
 }  // namespace scigap
