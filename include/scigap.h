@@ -163,6 +163,55 @@ int scigap_solve(const scigap_params* p,
                  void* workspace, size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------- */
+/* Batched solves (mixed-size queue)                                      */
+/* ---------------------------------------------------------------------- */
+
+/* One independent measurement of a batch.  The reference solves coded frames
+ * and measurement blocks one after another in Python loops
+ * (pnp_sci_algo.py:498, pnp_sci_test_data_realexp_blkproc.py:93); a batch
+ * runs all of them together, each with its own H, W, T. */
+typedef struct scigap_batch_item {
+  int64_t H, W, T;
+  const float* y;        /* (H,W)                                  */
+  const float* phi;      /* (T,H,W)                                */
+  const float* phi_sum;  /* (H,W), taken as passed                 */
+  const float* x0;       /* (T,H,W) warm start or NULL for At(y)   */
+  float* x_out;          /* (T,H,W) result                         */
+} scigap_batch_item;
+
+typedef struct scigap_batch_options {
+  int use_graph;   /* 1: capture the whole outer loop in a hipGraph on the
+                      first run and replay it afterwards; 0: plain launches */
+  int tile_rows;   /* waves per workgroup of the fused kernel for
+                      tv_iter_max = 5 (8, 12 or 16); 0 = 16               */
+  int grid;        /* persistent workgroups (0 = 256, one per CU)         */
+  int order;       /* 0: tiles of the most frames first; 1: submission order */
+} scigap_batch_options;
+
+typedef struct scigap_batch_plan scigap_batch_plan;  /* opaque */
+
+/* Workspace for a batch of n solves that share the solver parameters p
+ * (p->H, p->W, p->T are ignored; the items carry their own shapes). */
+size_t scigap_batch_workspace_bytes(const scigap_params* p, const scigap_batch_item* items,
+                                    int n, const scigap_batch_options* opt);
+
+/* Build a plan: per-iteration launch descriptors and tile queues are written
+ * into `workspace` (device memory the caller keeps alive with the plan).
+ * Supports tv_iter_max in [2, 7] (the fused depth). opt may be NULL. */
+int scigap_batch_plan_create(scigap_batch_plan** plan, const scigap_params* p,
+                             const scigap_batch_item* items, int n, void* workspace,
+                             size_t workspace_bytes, const scigap_batch_options* opt,
+                             void* stream);
+
+/* Run every solve of the plan to completion on `stream` (asynchronous).  The
+ * first run with use_graph captures the launches into a hipGraph bound to
+ * the plan's buffers; later runs replay it. */
+int scigap_batch_run(scigap_batch_plan* plan, void* stream);
+
+/* Free the plan (host objects and the graph; the workspace stays the caller's). */
+int scigap_batch_plan_destroy(scigap_batch_plan* plan);
+
+/* ---------------------------------------------------------------------- */
 /* Kernel timing (measurement support; no reference counterpart)          */
 /* ---------------------------------------------------------------------- */
 

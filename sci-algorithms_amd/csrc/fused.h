@@ -8,6 +8,9 @@
 
 namespace scigap {
 namespace fused {
+// Internal linkage: each translation unit that includes this header gets its
+// own copy of these kernels.
+namespace {
 
 
 // TV_ONLY runs the stencil alone (denoise_tv_chambolle): img = x_in, no
@@ -320,34 +323,26 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
   if constexpr (IQA) commit_err<BY>(err, a.err, red64, lp.w, lp.lane);
 }
 
-template <int D, int BY, int MR, int MODE, bool INSIDE, bool IQA>
+// Whether a tile's load region lies entirely in the image (mirrors TilePlace).
+template <int D, int BY, int MR>
+__host__ __device__ inline bool tile_inside(int tile, int tiles_x, int H, int W) {
+  using G = TileGeom<D, BY, MR>;
+  const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+  const int c0 = tx * G::OX - G::HC, r0 = ty * G::OY - D;
+  return c0 >= 0 && r0 >= 0 && c0 + G::LW <= W && r0 + G::LH <= H && (W % 2 == 0);
+}
+
+// One launch covers interior and border tiles; the two bodies are separate
+// instantiations (the interior one has no masks or bounds checks).
+template <int D, int BY, int MR, int MODE, bool IQA>
 __global__ void __launch_bounds__(64 * BY) k_fused(FusedArgs a) {
   __shared__ FusedLds<D, BY, MR> sh;
   const int tile = tile_of_block(blockIdx.x, a.per_xcd);
   if (tile >= a.ntiles) return;
-  fused_tile<D, BY, MR, MODE, INSIDE, IQA>(a, tile, sh);
-}
-
-// Batched form: a persistent grid pulls (solve, tile) entries from a queue
-// with an atomic counter until it is drained.  Each solve has its own
-// FusedArgs (shape, buffers, parameters); the queue is ordered by the host
-// (largest work first) so that the long tiles start early.
-template <int D, int BY, int MR, int MODE, bool INSIDE>
-__global__ void __launch_bounds__(64 * BY) k_fused_batch(const FusedArgs* __restrict__ descs,
-                                                        const int2* __restrict__ queue, int nq,
-                                                        int* counter) {
-  __shared__ FusedLds<D, BY, MR> sh;
-  __shared__ int s_item;
-  for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
-    __syncthreads();
-    const int q = s_item;
-    __syncthreads();
-    if (q >= nq) break;
-    const int2 it = queue[q];
-    fused_tile<D, BY, MR, MODE, INSIDE, false>(descs[it.x], it.y, sh);
-    __syncthreads();
-  }
+  if (tile_inside<D, BY, MR>(tile, a.tiles_x, a.H, a.W))
+    fused_tile<D, BY, MR, MODE, true, IQA>(a, tile, sh);
+  else
+    fused_tile<D, BY, MR, MODE, false, IQA>(a, tile, sh);
 }
 
 // Early-exit decision: one workgroup derives every frame's exit iteration
@@ -377,17 +372,20 @@ __global__ void __launch_bounds__(256) k_decide(const double* __restrict__ energ
 // contribution to yb is corrected by Phi*(x_new - x_old) (ADMM: 2*Phi*...),
 // and the residual epilogue is recomputed from the corrected yb.
 template <int D, int BY, int MR>
-__global__ void __launch_bounds__(64 * BY) k_fused_fix(FusedArgs a) {
+struct FixLds {
+  TileLds<BY> eng;
+  float2 img[TileGeom<D, BY, MR>::LH * 64];
+  double red64[BY];
+};
+
+template <int D, int BY, int MR>
+__device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, FixLds<D, BY, MR>& sh) {
   using G = TileGeom<D, BY, MR>;
   constexpr int RY = G::RY;
   constexpr int NH = MR;
-  __shared__ TileLds<BY> lds;
-  __shared__ float2 s_img[G::LH * 64];
-  __shared__ double red64[BY];
-
-  if (*a.nexit == 0) return;  // the common case: one scalar load per workgroup
-  const int tile = tile_of_block(blockIdx.x, a.per_xcd);
-  if (tile >= a.ntiles) return;
+  TileLds<BY>& lds = sh.eng;
+  float2* const s_img = sh.img;
+  double* const red64 = sh.red64;
   const int H = a.H, W = a.W, T = a.T;
   const int N = H * W;
   const TilePlace<D, BY, MR> tp(tile, a.tiles_x, H, W);
@@ -490,6 +488,15 @@ __global__ void __launch_bounds__(64 * BY) k_fused_fix(FusedArgs a) {
       }
   }
   if (a.truth && a.err && a.mode != ADMM && !tv_only) commit_err<BY>(err, a.err, red64, lp.w, lp.lane);
+}
+
+template <int D, int BY, int MR>
+__global__ void __launch_bounds__(64 * BY) k_fused_fix(FusedArgs a) {
+  __shared__ FixLds<D, BY, MR> sh;
+  if (*a.nexit == 0) return;  // the common case: one scalar load per workgroup
+  const int tile = tile_of_block(blockIdx.x, a.per_xcd);
+  if (tile >= a.ntiles) return;
+  fused_fix_tile<D, BY, MR>(a, tile, sh);
 }
 
 // Initial residual: X_0 = x0 (or At(y)), yb = A(X_0), then
@@ -667,5 +674,86 @@ __global__ void k_fill_i32(int* p, int n, int v) {
 }
 
 
+// ---------------------------------------------------------------------------
+// Host side: per-iteration launch arguments (shared by the single-solve and
+// batched drivers)
+// ---------------------------------------------------------------------------
+
+// Device buffers of one solve.
+struct SolveState {
+  float* xbuf = nullptr;     // (T,H,W) state buffer paired with x_out
+  float* bbuf = nullptr;     // (T,H,W) ADMM b
+  float* r[2] = {nullptr, nullptr};
+  float* y1[2] = {nullptr, nullptr};
+  float* yb = nullptr;       // (H,W)
+  int* stop = nullptr;       // [T]
+  int* nexit = nullptr;      // [iters]
+  double* energy = nullptr;  // [iters][T][D]
+};
+
+inline int solver_mode(const scigap_params& p) {
+  return p.method == SCIGAP_METHOD_ADMM ? ADMM : (p.accelerate ? GAP_ACC : GAP_PLAIN);
+}
+
+// Outer iterations that run the fused kernel: ADMM's last iteration is a
+// pointwise projection (its TV never reaches the returned x).
+inline int fused_iterations(const scigap_params& p) {
+  return solver_mode(p) == ADMM ? p.iters - 1 : p.iters;
+}
+
+inline double decayed(double v, double f, int k) {
+  for (int i = 0; i < k; ++i) v *= f;  // the reference multiplies once per iteration
+  return v;
+}
+
+// Buffer holding state X_j (GAP: x, ADMM: theta) so that the last state
+// lands in x_out; X_0 is the caller's x0 (or At(y), formed on the fly).
+inline float* state_buffer(const scigap_params& p, const SolveState& st, float* x_out, int j) {
+  return ((fused_iterations(p) - j) % 2 == 0) ? x_out : st.xbuf;
+}
+
+inline FusedArgs iteration_args(const scigap_params& p, int k, int64_t H, int64_t W, int64_t T,
+                                 const float* y, const float* phi, const float* phi_sum,
+                                 const float* x0, float* x_out, const float* truth,
+                                 double* sq_err, int* stats, const SolveState& st) {
+  const int K = p.iters, mode = solver_mode(p), D = p.tv_iter_max - 1;
+  const int last_fused = fused_iterations(p) - 1;
+  FusedArgs a{};
+  a.from_y = (k == 0 && !x0);
+  a.x_in = k == 0 ? x0 : state_buffer(p, st, x_out, k);
+  a.x_out = state_buffer(p, st, x_out, k + 1);
+  a.phi = phi;
+  a.y = y;
+  a.phi_sum = phi_sum;
+  a.r_in = st.r[k & 1];
+  a.r_out = st.r[(k + 1) & 1];
+  a.y1_in = st.y1[k & 1];
+  a.y1_out = st.y1[(k + 1) & 1];
+  a.truth = truth;
+  a.err = truth ? sq_err + k : nullptr;
+  a.b = st.bbuf;
+  a.energy = st.energy + size_t(k) * T * (D > 0 ? D : 1);
+  a.yb = st.yb;
+  a.stop = st.stop;
+  a.nexit = st.nexit + k;
+  a.stats = stats ? stats + k : nullptr;
+  a.H = int(H);
+  a.W = int(W);
+  a.T = int(T);
+  a.lam = p.lambda_;
+  a.gam_next = float(decayed(p.gamma, p.gamma_decay, k + 1));
+  const double wk = decayed(p.tv_weight, p.tv_weight_decay, k);
+  a.weight = float(wk);
+  a.tw = float(0.25 / wk);
+  a.eps = p.tv_eps;
+  a.mode = mode;
+  a.clip = p.clip;
+  a.write_b = mode == ADMM && (truth != nullptr || k == last_fused);
+  a.read_b = mode == ADMM && truth != nullptr && k > 0;
+  a.epilogue = (k + 1 < K);
+  return a;
+}
+
+}  // namespace
 }  // namespace fused
 }  // namespace scigap

@@ -24,11 +24,11 @@
 // iteration before it (and every iteration when per-iteration errors are
 // requested, since x_k = v_k + b_k).
 //
-// Early exit.  Each launch accumulates the per-frame energies E_0..E_{D-1}.
-// A second launch of the same kernel with FIX=true reads them; when no frame
-// stopped early every workgroup returns at once, otherwise it recomputes the
-// frames that stopped at iteration s < D from the unchanged inputs (x, r are
-// double-buffered), rewrites them, and recomputes the epilogue for its tile.
+// Early exit.  Each launch accumulates the per-frame energies E_0..E_{D-1};
+// a one-workgroup decision kernel derives each frame's exit iteration, and a
+// repair kernel (which returns at once when no frame stopped early)
+// recomputes the frames that stopped at s < D from the unchanged inputs
+// (x, r and y1 are double-buffered) and corrects yb and the epilogue.
 
 #include "fused.h"
 
@@ -45,9 +45,9 @@ using namespace fused;
 
 constexpr int kMaxFused = 6;
 
-// Workgroup shape per fused depth: 8 waves (512 threads, so the stencil may
-// use up to 256 VGPRs per lane); two strips of D rows per lane while D <= 4,
-// one strip beyond.  Load tile 128 x 8*MR*D, interior (128-2*HC) x (8*MR*D-2D).
+// Workgroup shape per fused depth: 16 waves (1024 lanes) of one D-row strip
+// each while D <= 4, 8 waves beyond (the strips get taller).  Load tile
+// 128 x BY*D, interior (128-2*HC) x (BY*D-2D).
 template <int D> struct FusedShape {
   static constexpr int BY = D <= 4 ? 16 : 8;
   static constexpr int MR = 1;
@@ -57,15 +57,10 @@ template <int D, int MODE, bool IQA>
 int launch_main(const FusedArgs& a, unsigned grid, hipStream_t s) {
   constexpr int BY = FusedShape<D>::BY, MR = FusedShape<D>::MR;
   {
-    ProfScope ps(MODE == TV_ONLY ? "tv_fused_interior" : "fused_interior", s);
-    k_fused<D, BY, MR, MODE, true, IQA><<<grid, 64 * BY, 0, s>>>(a);
+    ProfScope ps(MODE == TV_ONLY ? "tv_fused" : "fused_iteration", s);
+    k_fused<D, BY, MR, MODE, IQA><<<grid, 64 * BY, 0, s>>>(a);
   }
-  SCIGAP_LAUNCH_CHECK("fused iteration launch (interior tiles)");
-  {
-    ProfScope ps(MODE == TV_ONLY ? "tv_fused_border" : "fused_border", s);
-    k_fused<D, BY, MR, MODE, false, IQA><<<grid, 64 * BY, 0, s>>>(a);
-  }
-  SCIGAP_LAUNCH_CHECK("fused iteration launch (border tiles)");
+  SCIGAP_LAUNCH_CHECK("fused iteration launch");
   return SCIGAP_OK;
 }
 
@@ -120,17 +115,11 @@ unsigned grid_pixels(int64_t N) {
   return unsigned(b < 4096 ? (b < 1 ? 1 : b) : 4096);
 }
 
+// Workspace of one solve: SolveState plus the deep-TV buffers.
 struct Workspace {
-  float* xbuf = nullptr;  // T*N
-  float* bbuf = nullptr;  // T*N (ADMM)
+  SolveState st;
   float* vbuf = nullptr;  // T*N TV input (deep-TV path)
   void* tvws = nullptr;   // general TV workspace (deep-TV path)
-  float* yb = nullptr;    // (H,W) yb of the latest fused launch
-  int* stop = nullptr;    // [T] exit iterations of the latest fused launch
-  int* nexit = nullptr;   // [iters] early-exit counts
-  float* r[2] = {nullptr, nullptr};
-  float* y1[2] = {nullptr, nullptr};
-  double* energy = nullptr;  // iters * T * D
   size_t bytes = 0;
 };
 
@@ -138,21 +127,22 @@ Workspace carve(const scigap_params* p, char* base) {
   Workspace w;
   const size_t N = size_t(p->H) * p->W, T = size_t(p->T);
   const int D = p->tv_iter_max - 1 > 0 ? p->tv_iter_max - 1 : 1;
+  const size_t K = size_t(p->iters > 0 ? p->iters : 1);
   size_t off = 0;
   auto take = [&](size_t bytes) {
     char* q = base ? base + off : nullptr;
     off += align256(bytes);
     return q;
   };
-  w.xbuf = reinterpret_cast<float*>(take(T * N * 4));
-  if (p->method == SCIGAP_METHOD_ADMM) w.bbuf = reinterpret_cast<float*>(take(T * N * 4));
-  for (int i = 0; i < 2; ++i) w.r[i] = reinterpret_cast<float*>(take(N * 4));
+  w.st.xbuf = reinterpret_cast<float*>(take(T * N * 4));
+  if (p->method == SCIGAP_METHOD_ADMM) w.st.bbuf = reinterpret_cast<float*>(take(T * N * 4));
+  for (int i = 0; i < 2; ++i) w.st.r[i] = reinterpret_cast<float*>(take(N * 4));
   if (p->method == SCIGAP_METHOD_GAP && p->accelerate)
-    for (int i = 0; i < 2; ++i) w.y1[i] = reinterpret_cast<float*>(take(N * 4));
-  w.energy = reinterpret_cast<double*>(take(size_t(p->iters > 0 ? p->iters : 1) * T * D * 8));
-  w.yb = reinterpret_cast<float*>(take(N * 4));
-  w.stop = reinterpret_cast<int*>(take(T * 4));
-  w.nexit = reinterpret_cast<int*>(take(size_t(p->iters > 0 ? p->iters : 1) * 4));
+    for (int i = 0; i < 2; ++i) w.st.y1[i] = reinterpret_cast<float*>(take(N * 4));
+  w.st.energy = reinterpret_cast<double*>(take(K * T * D * 8));
+  w.st.yb = reinterpret_cast<float*>(take(N * 4));
+  w.st.stop = reinterpret_cast<int*>(take(T * 4));
+  w.st.nexit = reinterpret_cast<int*>(take(K * 4));
   if (p->tv_iter_max - 1 > kMaxFused) {
     w.vbuf = reinterpret_cast<float*>(take(T * N * 4));
     w.tvws = take(tv_general_workspace(p->H, p->W, p->T, p->tv_iter_max));
@@ -160,6 +150,8 @@ Workspace carve(const scigap_params* p, char* base) {
   w.bytes = off;
   return w;
 }
+
+}  // namespace
 
 int check_params(const scigap_params* p) {
   if (!p) return fail(SCIGAP_ENULL, "params is NULL");
@@ -169,10 +161,10 @@ int check_params(const scigap_params* p) {
   if (p->iters < 0) return fail(SCIGAP_EINVAL, "iters must be >= 0");
   if (p->tv_iter_max < 1) return fail(SCIGAP_EINVAL, "tv_iter_max must be >= 1");
   if (!(p->tv_weight > 0.f)) return fail(SCIGAP_EINVAL, "tv_weight must be > 0");
+  if (p->H * p->W >= (int64_t(1) << 30))
+    return fail(SCIGAP_EINVAL, "H*W must be < 2^30 (buffer-addressed planes)");
   return SCIGAP_OK;
 }
-
-}  // namespace
 
 }  // namespace scigap
 
@@ -227,12 +219,9 @@ int scigap_solve(const scigap_params* p, const float* y, const float* phi, const
 
   const int D = p->tv_iter_max - 1;
   const bool deep = D > kMaxFused;
-  if (N >= (int64_t(1) << 30))
-    return fail(SCIGAP_EINVAL, "H*W must be < 2^30 (buffer-addressed planes)");
-
   hipStream_t s = as_stream(stream);
   const int K = p->iters;
-  const int mode = p->method == SCIGAP_METHOD_ADMM ? ADMM : (p->accelerate ? GAP_ACC : GAP_PLAIN);
+  const int mode = solver_mode(*p);
   if (K == 0) {
     // No iterations: x = x0 or At(y).
     if (x0) {
@@ -241,75 +230,25 @@ int scigap_solve(const scigap_params* p, const float* y, const float* phi, const
     }
     return scigap_adjoint(y, phi, x_out, p->H, p->W, T, stream);
   }
-  if (!deep)
-    if (int rc = fill_f64(ws.energy, K * int(T) * D, 0.0, s)) return rc;
+  if (!deep && D > 0)
+    if (int rc = fill_f64(ws.st.energy, K * int(T) * D, 0.0, s)) return rc;
   if (truth)
     if (int rc = fill_f64(sq_err, K, 0.0, s)) return rc;
   if (stats) {
     k_fill_i32<<<(K + 255) / 256, 256, 0, s>>>(stats, K, 0);
     SCIGAP_LAUNCH_CHECK("stats fill");
   }
-
-  // gamma/tv_weight schedule (decays as in ADMM_TV_rec, pnp_sci_algo.py:898-899).
-  auto gamma_at = [&](int k) {
-    double g = p->gamma;
-    for (int i = 0; i < k; ++i) g *= double(p->gamma_decay);
-    return g;
-  };
-  auto weight_at = [&](int k) {
-    double w = p->tv_weight;
-    for (int i = 0; i < k; ++i) w *= double(p->tv_weight_decay);
-    return w;
-  };
-
   {
     ProfScope ps("init", s);
-    k_init<<<grid_pixels(N), 256, 0, s>>>(x0, phi, y, phi_sum, ws.r[0], ws.y1[0], N, int(T), mode,
-                                          float(gamma_at(0)));
+    k_init<<<grid_pixels(N), 256, 0, s>>>(x0, phi, y, phi_sum, ws.st.r[0], ws.st.y1[0], N, int(T),
+                                          mode, float(p->gamma));
   }
   SCIGAP_LAUNCH_CHECK("init launch");
 
-  const bool iqa = truth != nullptr;
-  // ADMM's TV of the last iteration never reaches the returned x.
-  const int fused_iters = mode == ADMM ? K - 1 : K;
-  // Buffer holding X_j (GAP: x, ADMM: theta): X_last lands in x_out.
-  const int last = fused_iters;  // index of the final state
-  auto state = [&](int j) -> float* { return ((last - j) % 2 == 0) ? x_out : ws.xbuf; };
-
+  const int fused_iters = fused_iterations(*p);
   for (int k = 0; k < fused_iters; ++k) {
-    FusedArgs a{};
-    a.from_y = (k == 0 && !x0);
-    a.x_in = k == 0 ? x0 : state(k);
-    a.x_out = state(k + 1);
-    a.phi = phi;
-    a.y = y;
-    a.phi_sum = phi_sum;
-    a.r_in = ws.r[k & 1];
-    a.r_out = ws.r[(k + 1) & 1];
-    a.y1_in = ws.y1[k & 1];
-    a.y1_out = ws.y1[(k + 1) & 1];
-    a.truth = truth;
-    a.err = iqa ? sq_err + k : nullptr;
-    a.b = ws.bbuf;
-    a.energy = ws.energy + size_t(k) * T * D;
-    a.yb = ws.yb;
-    a.stop = ws.stop;
-    a.nexit = ws.nexit + k;
-    a.stats = stats ? stats + k : nullptr;
-    a.H = int(p->H);
-    a.W = int(p->W);
-    a.T = int(T);
-    a.lam = p->lambda_;
-    a.gam_next = float(gamma_at(k + 1));
-    double wk = weight_at(k);
-    a.weight = float(wk);
-    a.tw = float(0.25 / wk);
-    a.eps = p->tv_eps;
-    a.mode = mode;
-    a.clip = p->clip;
-    a.write_b = mode == ADMM && (iqa || k == fused_iters - 1);
-    a.read_b = mode == ADMM && iqa && k > 0;
-    a.epilogue = (k + 1 < K);
+    FusedArgs a = iteration_args(*p, k, p->H, p->W, T, y, phi, phi_sum, x0, x_out, truth, sq_err,
+                                 stats, ws.st);
     if (D == 0) {
       k_project<<<grid_pixels(N), 256, 0, s>>>(a);
       SCIGAP_LAUNCH_CHECK("projection launch");
@@ -333,10 +272,11 @@ int scigap_solve(const scigap_params* p, const float* y, const float* phi, const
   if (mode == ADMM) {
     // x_{K-1} = theta_{K-1} + b_{K-1} + lambda*r_{K-1}*Phi.
     const bool fy = (K == 1 && !x0);
-    const float* th = K == 1 ? x0 : state(K - 1);
-    k_admm_final<<<grid_pixels(N), 256, 0, s>>>(th, K == 1 ? nullptr : ws.bbuf, ws.r[(K - 1) & 1],
-                                                phi, y, x_out, truth, iqa ? sq_err + (K - 1) : nullptr,
-                                                N, int(T), p->lambda_, fy ? 1 : 0);
+    const float* th = K == 1 ? x0 : state_buffer(*p, ws.st, x_out, K - 1);
+    k_admm_final<<<grid_pixels(N), 256, 0, s>>>(th, K == 1 ? nullptr : ws.st.bbuf,
+                                                ws.st.r[(K - 1) & 1], phi, y, x_out, truth,
+                                                truth ? sq_err + (K - 1) : nullptr, N, int(T),
+                                                p->lambda_, fy ? 1 : 0);
     SCIGAP_LAUNCH_CHECK("admm final launch");
   }
   return SCIGAP_OK;

@@ -13,7 +13,8 @@ import threading
 
 from .build import LIB_PATH
 
-__all__ = ["ScigapError", "ScigapUnavailable", "Params", "KernelStat", "lib", "check",
+__all__ = ["ScigapError", "ScigapUnavailable", "Params", "KernelStat", "BatchItem",
+           "BatchOptions", "lib", "check",
            "stream_ptr", "kernel_profile", "SYMBOLS", "METHOD_GAP", "METHOD_ADMM"]
 
 METHOD_GAP = 0
@@ -50,6 +51,21 @@ class Params(ctypes.Structure):
     ]
 
 
+class BatchItem(ctypes.Structure):
+    """Mirror of ``scigap_batch_item``."""
+
+    _fields_ = [("H", ctypes.c_int64), ("W", ctypes.c_int64), ("T", ctypes.c_int64),
+                ("y", ctypes.c_void_p), ("phi", ctypes.c_void_p), ("phi_sum", ctypes.c_void_p),
+                ("x0", ctypes.c_void_p), ("x_out", ctypes.c_void_p)]
+
+
+class BatchOptions(ctypes.Structure):
+    """Mirror of ``scigap_batch_options``."""
+
+    _fields_ = [("use_graph", ctypes.c_int), ("tile_rows", ctypes.c_int), ("grid", ctypes.c_int),
+                ("order", ctypes.c_int)]
+
+
 class KernelStat(ctypes.Structure):
     """Mirror of ``scigap_kernel_stat``."""
 
@@ -79,6 +95,14 @@ SYMBOLS = {
     "scigap_solve_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(Params)]),
     "scigap_solve": (_INT, [ctypes.POINTER(Params), _P, _P, _P, _P, _P, _P, _P, _P, _P,
                             ctypes.c_size_t, _P]),
+    "scigap_batch_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(Params),
+                                                       ctypes.POINTER(BatchItem), _INT,
+                                                       ctypes.POINTER(BatchOptions)]),
+    "scigap_batch_plan_create": (_INT, [ctypes.POINTER(_P), ctypes.POINTER(Params),
+                                        ctypes.POINTER(BatchItem), _INT, _P, ctypes.c_size_t,
+                                        ctypes.POINTER(BatchOptions), _P]),
+    "scigap_batch_run": (_INT, [_P, _P]),
+    "scigap_batch_plan_destroy": (_INT, [_P]),
     "scigap_profile_begin": (_INT, []),
     "scigap_profile_end": (_INT, [ctypes.POINTER(KernelStat), _INT, ctypes.POINTER(_INT)]),
 }

@@ -1,0 +1,107 @@
+"""Batched solves (mixed-size queue): against single solves and the FP64 oracle.
+
+Every result of a batch must match the same problem solved alone through
+scigap_solve (same kernels; only the order of the fp64 energy atomics can
+differ) to 1e-6 relative, and the FP64 oracle to 1e-4.  hipGraph replays
+are each checked against an eager run, with eager work in between.
+"""
+
+import numpy as np
+import pytest
+
+import pnp_sci_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(48, 80, 5), (64, 64, 8), (100, 150, 6), (33, 47, 4), (130, 260, 8), (20, 24, 3)]
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _problems(seed=0):
+    from pnp_sci.synthetic import make_problem
+    return [make_problem(H, W, T, seed=seed + i, p=0.3 + 0.08 * i) for i, (H, W, T) in
+            enumerate(SIZES)]
+
+
+def _device(pr):
+    from pnp_sci.device import to_device
+    return (to_device(pr.y), to_device(pr.mask_thw), to_device(pr.phi_sum), None)
+
+
+@pytest.mark.parametrize("method,kw", [("gap", {}), ("gap", {"accelerate": False}),
+                                       ("admm", {}), ("admm", {"gamma": 0.0, "clip": True}),
+                                       ("gap", {"tv_iter_max": 3}), ("gap", {"tv_iter_max": 7})])
+def test_batch_matches_single_and_oracle(gpu, method, kw):
+    from pnp_sci.batch import BatchPlan
+    from pnp_sci.pnp_sci_algo import solve_device
+    probs = _problems()
+    dev = [_device(p) for p in probs]
+    iters = 10
+    plan = BatchPlan(dev, method=method, iters=iters, tv_weight=0.1, use_graph=False, **kw)
+    xs = [x.cpu().numpy() for x in plan.run()]
+    plan.close()
+    for pr, d, xb in zip(probs, dev, xs):
+        xs1, _, _ = solve_device(d[0], d[1], d[2], method=method, iters=iters, tv_weight=0.1, **kw)
+        assert rel(xb, xs1.cpu().numpy()) <= 1e-6
+        m = pr.mask.astype(np.float64)
+        A = lambda v: O.A_(v, m)  # noqa: E731
+        At = lambda v: O.At_(v, m)  # noqa: E731
+        okw = dict(kw)
+        if method == "gap":
+            want = O.gap_denoise(pr.y.astype(np.float64), pr.phi_sum.astype(np.float64), A, At,
+                                 iter_max=iters, tv_weight=0.1, show_iqa=False, **okw)[0]
+        else:
+            want = O.admm_denoise(pr.y.astype(np.float64), pr.phi_sum.astype(np.float64), A, At,
+                                  iter_max=iters, tv_weight=0.1, show_iqa=False, **okw)[0]
+        assert rel(np.moveaxis(xb, 0, -1), want) <= 1e-4
+
+
+@pytest.mark.parametrize("tile_rows", [8, 12, 16])
+def test_batch_graph_replays_and_tile_shapes(gpu, tile_rows):
+    import torch
+    from pnp_sci.batch import BatchPlan
+    probs = _problems(seed=7)
+    dev = [_device(p) for p in probs]
+    eager = BatchPlan(dev, iters=8, tv_weight=0.1, use_graph=False, tile_rows=tile_rows)
+    ref = [x.clone() for x in eager.run()]
+    graph = BatchPlan(dev, iters=8, tv_weight=0.1, use_graph=True, tile_rows=tile_rows)
+    for rep in range(3):
+        got = graph.run()
+        torch.cuda.synchronize()
+        for a, b in zip(got, ref):
+            assert float((a - b).abs().max()) <= 1e-5 * float(b.abs().max()), rep
+        # eager work on the same stream between replays
+        again = eager.run()
+        torch.cuda.synchronize()
+        for a, b in zip(again, ref):
+            assert float((a - b).abs().max()) <= 1e-5 * float(b.abs().max())
+    eager.close()
+    graph.close()
+
+
+def test_gap_denoise_batch_reference_layout(gpu):
+    from pnp_sci.batch import gap_denoise_batch
+    probs = _problems(seed=3)[:3]
+    xs = gap_denoise_batch([p.y for p in probs], [np.ascontiguousarray(p.mask) for p in probs],
+                           [p.phi_sum for p in probs], iter_max=6, tv_weight=0.1)
+    for pr, x in zip(probs, xs):
+        m = pr.mask.astype(np.float64)
+        want = O.gap_denoise(pr.y.astype(np.float64), pr.phi_sum.astype(np.float64),
+                             lambda v: O.A_(v, m), lambda v: O.At_(v, m), iter_max=6,
+                             tv_weight=0.1, show_iqa=False)[0]
+        assert x.shape == pr.mask.shape and rel(x, want) <= 1e-4
+
+
+def test_batch_rejects_unsupported(gpu):
+    from pnp_sci import _lib
+    from pnp_sci.batch import BatchPlan
+    dev = [_device(p) for p in _problems()[:2]]
+    with pytest.raises(_lib.ScigapError):
+        BatchPlan(dev, iters=4, tv_iter_max=12)   # deep TV is not batched
+    with pytest.raises(ValueError):
+        BatchPlan([], iters=4)
