@@ -68,7 +68,7 @@ __device__ __forceinline__ void st1(float* p, int N, unsigned o, float v) {
 
 // Load an RY x 2 register block of one plane (zero outside the image).
 // `off` is the plane offset of register (0,0).  INSIDE tiles lie entirely in
-// the image (and W is even): one 8-byte buffer load per row, the row step in
+// the image (and W is even): one 8-byte buffer load per row (two merged dword loads), the row step in
 // the scalar offset.  Border tiles send pixels outside the image to an offset
 // past the end of the buffer, which reads as 0.
 template <int RY, bool INSIDE>
@@ -79,9 +79,12 @@ __device__ __forceinline__ void load_plane(const float* __restrict__ src, float 
     const unsigned vo = unsigned(off) * 4u;
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
-      const auto q = __builtin_amdgcn_raw_buffer_load_b64(rs, vo, j * W * 4, 0);
-      v[j][0] = __builtin_bit_cast(float, q[0]);
-      v[j][1] = __builtin_bit_cast(float, q[1]);
+      // Two dword loads at adjacent offsets: the backend merges them into one
+      // buffer_load_dwordx2 (the 64-bit builtin lowers to a single dword load
+      // on this toolchain, so it is not used).
+      v[j][0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, vo, j * W * 4, 0));
+      v[j][1] = __builtin_bit_cast(float,
+                                   __builtin_amdgcn_raw_buffer_load_b32(rs, vo + 4u, j * W * 4, 0));
     }
   } else {
 #pragma unroll
