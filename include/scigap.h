@@ -133,6 +133,9 @@ typedef struct scigap_params {
                             0.999 in ADMM_TV_rec, pnp_sci_algo.py:898)   */
   float gamma_decay;   /* per-iteration factor on gamma (1: none;
                           0.998 in ADMM_TV_rec, pnp_sci_algo.py:899)     */
+  int tile_rows;       /* fused-kernel workgroup height in waves for
+                          tv_iter_max = 5: 8, 12 or 16 (0 = 16); a tuning
+                          knob with no effect on results                 */
 } scigap_params;
 
 /* Fill *p with the reference defaults for a GAP solve of an HxWxT problem:

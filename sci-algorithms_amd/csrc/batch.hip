@@ -482,11 +482,20 @@ int scigap_batch_run(scigap_batch_plan* P, void* stream) {
   hipStream_t s = as_stream(stream);
   if (!P->opt.use_graph) return enqueue_all(P, s);
   if (!P->exec) {
-    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return hip_fail(e, "begin graph capture");
-    int rc = enqueue_all(P, s);
+    // Capture on a private stream: the caller's stream may be the legacy
+    // null stream, which cannot be captured.  Capturing executes nothing.
+    hipStream_t cs = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "capture stream");
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      hipStreamDestroy(cs);
+      return hip_fail(e, "begin graph capture");
+    }
+    int rc = enqueue_all(P, cs);
     hipGraph_t g = nullptr;
-    e = hipStreamEndCapture(s, &g);
+    e = hipStreamEndCapture(cs, &g);
+    hipStreamDestroy(cs);
     if (rc) {
       if (g) hipGraphDestroy(g);
       return rc;

@@ -171,7 +171,7 @@ struct FusedLds {
   static constexpr int LH = TileGeom<D, BY, MR>::LH;
   TileLds<BY> eng;
   float2 q[LH * 64], ph[LH * 64], yb[LH * 64], img[LH * 64];
-  float red[BY * D];
+  float red[kEnergyFrames * D];  // per-frame energy partials of this tile
   double red64[BY];
 };
 
@@ -221,6 +221,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
 #pragma unroll
     for (int j = 0; j < RY; ++j) s_yb[sl0 + j * 64] = make_float2(0.f, 0.f);
   }
+  for (int i = threadIdx.x; i < kEnergyFrames * D; i += blockDim.x) red[i] = 0.f;
   float err = 0.f;
 
   // Frame t+1's x and Phi are requested before frame t's TV so that their
@@ -299,7 +300,11 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
       }
       s_yb[sl0 + j * 64] = yb;
     }
-    energy_commit<D, BY, MR, NH>(acc, tp.lane_owner, en, red, lp);
+    const int tb = t % kEnergyFrames;
+    energy_add<D, BY, MR, NH>(acc, tp.lane_owner, red + tb * D, lp);
+    if (tb == kEnergyFrames - 1 || t == T - 1) {
+      energy_flush<D>(red, tb + 1, en - size_t(tb) * D);
+    }
     phit += N;
     xt += N;
     xo += N;

@@ -53,38 +53,56 @@ template <int D> struct FusedShape {
   static constexpr int MR = 1;
 };
 
-template <int D, int MODE, bool IQA>
-int launch_main(const FusedArgs& a, unsigned grid, hipStream_t s) {
-  constexpr int BY = FusedShape<D>::BY, MR = FusedShape<D>::MR;
+// Tile grid of the (D, BY) geometry written into the launch arguments.
+template <int D, int BY, int MR>
+unsigned set_tiles(FusedArgs& a) {
+  using G = TileGeom<D, BY, MR>;
+  a.tiles_x = (a.W + G::OX - 1) / G::OX;
+  const int tiles_y = (a.H + G::OY - 1) / G::OY;
+  a.ntiles = a.tiles_x * tiles_y;
+  a.per_xcd = (a.ntiles + 7) / 8;
+  return unsigned(a.per_xcd * 8);
+}
+
+template <int D, int BY, int MODE, bool IQA>
+int launch_main(const FusedArgs& a0, hipStream_t s) {
+  FusedArgs a = a0;
+  const unsigned grid = set_tiles<D, BY, 1>(a);
   {
     ProfScope ps(MODE == TV_ONLY ? "tv_fused" : "fused_iteration", s);
-    k_fused<D, BY, MR, MODE, IQA><<<grid, 64 * BY, 0, s>>>(a);
+    k_fused<D, BY, 1, MODE, IQA><<<grid, 64 * BY, 0, s>>>(a);
   }
   SCIGAP_LAUNCH_CHECK("fused iteration launch");
   return SCIGAP_OK;
 }
 
-template <int D>
-int launch_fused(const FusedArgs& a0, hipStream_t s) {
-  constexpr int BY = FusedShape<D>::BY, MR = FusedShape<D>::MR;
-  using G = TileGeom<D, BY, MR>;
-  FusedArgs a = a0;
-  a.tiles_x = (a.W + G::OX - 1) / G::OX;
-  int tiles_y = (a.H + G::OY - 1) / G::OY;
-  a.ntiles = a.tiles_x * tiles_y;
-  a.per_xcd = (a.ntiles + 7) / 8;
-  unsigned grid = unsigned(a.per_xcd * 8);
+template <int D, int BY>
+int launch_mode(const FusedArgs& a, hipStream_t s) {
   const bool iqa = a.truth != nullptr;
-  int rc;
   if (a.mode == GAP_ACC)
-    rc = iqa ? launch_main<D, GAP_ACC, true>(a, grid, s) : launch_main<D, GAP_ACC, false>(a, grid, s);
-  else if (a.mode == GAP_PLAIN)
-    rc = iqa ? launch_main<D, GAP_PLAIN, true>(a, grid, s) : launch_main<D, GAP_PLAIN, false>(a, grid, s);
-  else if (a.mode == ADMM)
-    rc = iqa ? launch_main<D, ADMM, true>(a, grid, s) : launch_main<D, ADMM, false>(a, grid, s);
-  else
-    rc = launch_main<D, TV_ONLY, false>(a, grid, s);
+    return iqa ? launch_main<D, BY, GAP_ACC, true>(a, s) : launch_main<D, BY, GAP_ACC, false>(a, s);
+  if (a.mode == GAP_PLAIN)
+    return iqa ? launch_main<D, BY, GAP_PLAIN, true>(a, s)
+               : launch_main<D, BY, GAP_PLAIN, false>(a, s);
+  if (a.mode == ADMM)
+    return iqa ? launch_main<D, BY, ADMM, true>(a, s) : launch_main<D, BY, ADMM, false>(a, s);
+  return launch_main<D, BY, TV_ONLY, false>(a, s);
+}
+
+template <int D>
+int launch_fused(const FusedArgs& a0, hipStream_t s, int tile_rows) {
+  constexpr int BY = FusedShape<D>::BY, MR = FusedShape<D>::MR;
+  int rc;
+  if constexpr (D == 4) {
+    rc = tile_rows == 8    ? launch_mode<4, 8>(a0, s)
+         : tile_rows == 12 ? launch_mode<4, 12>(a0, s)
+                           : launch_mode<4, 16>(a0, s);
+  } else {
+    rc = launch_mode<D, BY>(a0, s);
+  }
   if (rc) return rc;
+  FusedArgs a = a0;
+  const unsigned grid = set_tiles<D, BY, MR>(a);
   {
     ProfScope ps("fused_decide", s);
     k_decide<D><<<1, 256, 0, s>>>(a.energy, a.T, a.eps, a.stop, a.nexit, a.stats);
@@ -98,14 +116,14 @@ int launch_fused(const FusedArgs& a0, hipStream_t s) {
   return SCIGAP_OK;
 }
 
-int launch_fused_depth(int D, const FusedArgs& a, hipStream_t s) {
+int launch_fused_depth(int D, const FusedArgs& a, hipStream_t s, int tile_rows = 0) {
   switch (D) {
-    case 1: return launch_fused<1>(a, s);
-    case 2: return launch_fused<2>(a, s);
-    case 3: return launch_fused<3>(a, s);
-    case 4: return launch_fused<4>(a, s);
-    case 5: return launch_fused<5>(a, s);
-    case 6: return launch_fused<6>(a, s);
+    case 1: return launch_fused<1>(a, s, tile_rows);
+    case 2: return launch_fused<2>(a, s, tile_rows);
+    case 3: return launch_fused<3>(a, s, tile_rows);
+    case 4: return launch_fused<4>(a, s, tile_rows);
+    case 5: return launch_fused<5>(a, s, tile_rows);
+    case 6: return launch_fused<6>(a, s, tile_rows);
   }
   return fail(SCIGAP_EUNSUPPORTED, "fused depth %d not instantiated", D);
 }
@@ -161,6 +179,8 @@ int check_params(const scigap_params* p) {
   if (p->iters < 0) return fail(SCIGAP_EINVAL, "iters must be >= 0");
   if (p->tv_iter_max < 1) return fail(SCIGAP_EINVAL, "tv_iter_max must be >= 1");
   if (!(p->tv_weight > 0.f)) return fail(SCIGAP_EINVAL, "tv_weight must be > 0");
+  if (p->tile_rows != 0 && p->tile_rows != 8 && p->tile_rows != 12 && p->tile_rows != 16)
+    return fail(SCIGAP_EINVAL, "tile_rows must be 0, 8, 12 or 16");
   if (p->H * p->W >= (int64_t(1) << 30))
     return fail(SCIGAP_EINVAL, "H*W must be < 2^30 (buffer-addressed planes)");
   return SCIGAP_OK;
@@ -188,6 +208,7 @@ void scigap_default_params(scigap_params* p, int64_t H, int64_t W, int64_t T) {
   p->tv_iter_max = 5;
   p->tv_weight_decay = 1.f;
   p->gamma_decay = 1.f;
+  p->tile_rows = 0;
 }
 
 size_t scigap_solve_workspace_bytes(const scigap_params* p) {
@@ -265,7 +286,7 @@ int scigap_solve(const scigap_params* p, const float* y, const float* phi, const
       ea.x_in = ws.vbuf;
       k_gap_epi<<<grid_pixels(N), 256, 0, s>>>(ea);
       SCIGAP_LAUNCH_CHECK("epilogue launch");
-    } else if (int rc = launch_fused_depth(D, a, s)) {
+    } else if (int rc = launch_fused_depth(D, a, s, p->tile_rows)) {
       return rc;
     }
   }

@@ -204,14 +204,17 @@ __device__ __forceinline__ void tv_run_from_zero(const float2* img, float (&out)
   }
 }
 
-// Reduce the energy partials over the workgroup and add the totals to
-// energy[0..D) in fp64.  A lane contributes half h of its strip when those
-// rows are interior rows of the tile and its columns are interior columns.
+// Frame energies: each wave reduces its lanes' partials (DPP) and adds them
+// into an LDS slot of the frame with an LDS float atomic, so frames need no
+// barrier; energy_flush moves a block of frames to the global fp64
+// accumulators (one barrier and T*D atomics per workgroup per block).
+// A lane contributes half h of its strip when those rows are interior rows of
+// the tile and its columns are interior columns.
+constexpr int kEnergyFrames = 32;  // frames buffered in LDS between flushes
+
 template <int D, int BY, int MR, int NH>
-__device__ __forceinline__ void energy_commit(const float (&acc)[NH * 8], bool lane_owner,
-                                              double* energy, float* red /* LDS, BY*D */,
-                                              const LanePos& lp) {
-  // Which halves of this wave's strip are interior rows.
+__device__ __forceinline__ void energy_add(const float (&acc)[NH * 8], bool lane_owner,
+                                           float* slot /* LDS, D floats */, const LanePos& lp) {
   bool lo_in, hi_in;
   if (MR == 2) {
     lo_in = lp.w > 0;          // rows [0, D) of wave 0 are the top halo
@@ -219,6 +222,7 @@ __device__ __forceinline__ void energy_commit(const float (&acc)[NH * 8], bool l
   } else {
     lo_in = hi_in = lp.w > 0 && lp.w < BY - 1;
   }
+  if (!lo_in && !hi_in) return;  // wave-uniform: halo waves hold no interior rows
 #pragma unroll
   for (int u = 0; u < D; ++u) {
     float v = 0.f;
@@ -226,18 +230,21 @@ __device__ __forceinline__ void energy_commit(const float (&acc)[NH * 8], bool l
       v = lo_in ? acc[u] : 0.f;
       if (NH == 2) v += hi_in ? acc[8 + u] : 0.f;
     }
-    v = wave_sum(v);
-    if (lp.lane == 0) red[lp.w * D + u] = v;
+    v = wave_sum_dpp(v);
+    if (lp.lane == 0) atomicAdd(slot + u, v);
+  }
+}
+
+// Add frames [t0, t0+nt) of the LDS block to energy (fp64, [T][D]) and clear
+// the block.  All threads of the workgroup call it.
+template <int D>
+__device__ __forceinline__ void energy_flush(float* block, int nt, double* energy_t0) {
+  __syncthreads();
+  for (int i = threadIdx.x; i < nt * D; i += blockDim.x) {
+    atomicAdd(energy_t0 + i, double(block[i]));
+    block[i] = 0.f;
   }
   __syncthreads();
-  if (lp.w == 0 && lp.lane < D) {
-    double tot = 0.0;
-#pragma unroll
-    for (int q = 0; q < BY; ++q) tot += double(red[q * D + lp.lane]);
-    atomicAdd(energy + lp.lane, tot);
-  }
-  // The next writer of `red` is this function for the next frame, after at
-  // least two more barriers inside the engine.
 }
 
 // Early-exit iteration of one frame from its energies E_0..E_{D-1}
