@@ -134,7 +134,7 @@ typedef struct scigap_params {
   float gamma_decay;   /* per-iteration factor on gamma (1: none;
                           0.998 in ADMM_TV_rec, pnp_sci_algo.py:899)     */
   int tile_rows;       /* fused-kernel workgroup height in waves for
-                          tv_iter_max = 5: 8, 12 or 16 (0 = 16); a tuning
+                          tv_iter_max = 5: 8, 12 or 16 (0 = 12); a tuning
                           knob with no effect on results                 */
 } scigap_params;
 
@@ -186,7 +186,7 @@ typedef struct scigap_batch_options {
   int use_graph;   /* 1: capture the whole outer loop in a hipGraph on the
                       first run and replay it afterwards; 0: plain launches */
   int tile_rows;   /* waves per workgroup of the fused kernel for
-                      tv_iter_max = 5 (8, 12 or 16); 0 = 16               */
+                      tv_iter_max = 5 (8, 12 or 16); 0 = 12               */
   int grid;        /* persistent workgroups (0 = 256, one per CU)         */
   int order;       /* 0: tiles of the most frames first; 1: submission order */
 } scigap_batch_options;
@@ -213,6 +213,59 @@ int scigap_batch_run(scigap_batch_plan* plan, void* stream);
 
 /* Free the plan (host objects and the graph; the workspace stays the caller's). */
 int scigap_batch_plan_destroy(scigap_batch_plan* plan);
+
+/* ---------------------------------------------------------------------- */
+/* Differentiable unrolled GAP-TV                                         */
+/* ---------------------------------------------------------------------- */
+
+/* K outer iterations of accelerated GAP-TV (pnp_sci_algo.py:625-650, the
+ * loop of gap_denoise with denoiser='tv') unrolled as a fixed computation:
+ * iteration k uses its own lambda_k and tv_weight_k, and every TV call runs
+ * exactly tv_iter_max-1 Chambolle dual updates (scikit-image _denoise.py:
+ * 341-393 without the energy early exit, which would make the map
+ * discontinuous).  A batch of `batch` samples shares one mask.  The reference
+ * has no backward pass; its networks (BIRNAT, RevSCI-net) train in PyTorch. */
+typedef struct scigap_unrolled {
+  int64_t H, W, T;
+  int batch;        /* samples sharing the mask                               */
+  int iters;        /* K outer iterations, >= 1                               */
+  int tv_iter_max;  /* n in [2, 7]: n-1 dual updates per TV call              */
+  int logits;       /* 1: `mask` holds logits and phi = sigmoid(mask);
+                       0: `mask` holds phi itself                             */
+  int measure;      /* 1: `meas` holds truth videos (batch,T,H,W) and
+                       y_b = sum_t phi_t*truth_{b,t} is formed on the device;
+                       0: `meas` holds the measurements y (batch,H,W)         */
+} scigap_unrolled;
+
+/* Bytes of the checkpoint buffer one forward call fills and the matching
+ * backward call reads: per sample, x_k (T,H,W) and y1_k (H,W) at the start
+ * of every outer iteration, and y. */
+size_t scigap_unrolled_checkpoint_bytes(const scigap_unrolled* u);
+
+/* Workspace of a forward or a backward call (the same size for both). */
+size_t scigap_unrolled_workspace_bytes(const scigap_unrolled* u);
+
+/* Forward: x_out (batch,T,H,W) = x_K of every sample.  lambdas and weights
+ * are HOST arrays of K floats (weights > 0).  mask (T,H,W) per `logits`,
+ * meas per `measure`.  ckpt receives the per-iteration activations. */
+int scigap_unrolled_forward(const scigap_unrolled* u, const float* lambdas,
+                            const float* weights, const float* mask, const float* meas,
+                            float* x_out, void* ckpt, size_t ckpt_bytes,
+                            void* workspace, size_t workspace_bytes, void* stream);
+
+/* Backward of the same call, from grad_x (batch,T,H,W) = dL/dx_out:
+ *   grad_mask   (T,H,W) dL/dmask (logits or phi, as passed), or NULL
+ *   grad_meas   dL/dy (batch,H,W) when measure = 0, or NULL; with
+ *               measure = 1 the truth videos get no gradient (pass NULL)
+ *   grad_lambda, grad_weight  device arrays of K doubles, or NULL
+ * Every output is overwritten.  Inner TV states are recomputed per outer
+ * iteration from the checkpoint (hand-written reverse kernels). */
+int scigap_unrolled_backward(const scigap_unrolled* u, const float* lambdas,
+                             const float* weights, const float* mask, const float* meas,
+                             const void* ckpt, size_t ckpt_bytes, const float* grad_x,
+                             float* grad_mask, float* grad_meas, double* grad_lambda,
+                             double* grad_weight, void* workspace, size_t workspace_bytes,
+                             void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Kernel timing (measurement support; no reference counterpart)          */

@@ -166,7 +166,7 @@ Geom geom_of(int D, int BY) {
   return g;
 }
 
-int default_rows(int D) { return D <= 4 ? 16 : 8; }
+int default_rows(int D) { return D < 4 ? 16 : (D == 4 ? 12 : 8); }
 
 int pick_rows(int D, const scigap_batch_options* opt) {
   if (D == 4 && opt && (opt->tile_rows == 8 || opt->tile_rows == 12 || opt->tile_rows == 16))

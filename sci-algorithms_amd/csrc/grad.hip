@@ -1,0 +1,603 @@
+// Reverse-mode kernels of the unrolled, differentiable GAP-TV.
+//
+// Model (per sample; Phi is shared by the batch):
+//   y     = sum_t Phi_t * truth_t            (or a given measurement)
+//   S     = sum_t Phi_t
+//   x_0   = Phi * y,  y1_0 = 0
+//   for k < K:  yb = sum_t Phi_t x_k,  y1' = y1_k + (y - yb),
+//               r = (y1' - yb)/S,  v = x_k + lambda_k * r * Phi,
+//               x_{k+1} = TV_{w_k}(v)  (D = tv_iter_max-1 Chambolle updates,
+//                                        no early exit)
+// which is PnP_SCI/python/pnp_sci_algo.py:625-650 (accelerated GAP) with the
+// TV step of scikit-image 0.18.3 _denoise.py:341-393 run for a fixed number
+// of iterations, and per-iteration lambda_k / tv_weight_k.
+//
+// One Chambolle update, with G the forward difference (zero in the last row /
+// column) and G^T its transpose (the "d" of _denoise.py:352-361):
+//   out_u = v + G^T p_u,  g_u = G out_u,  n_u = |g_u|,  s_u = 1/(1 + c n_u),
+//   a_u = p_u - tau g_u,  p_{u+1} = a_u * s_u,   c = tau / w,  tau = 1/4.
+// Reverse of one update, given pbar_{u+1}:
+//   abar_u = pbar_{u+1} * s_u,   sbar = <pbar_{u+1}, a_u>,
+//   nbar = -c s_u^2 sbar,  cbar += -n_u s_u^2 sbar,
+//   gbar_u = -tau abar_u + nbar g_u / n_u   (0 where n_u = 0),
+//   obar_u = G^T gbar_u,  vbar += obar_u,  pbar_u = abar_u + G obar_u.
+// The TV output out_D = v + G^T p_D starts the chain: vbar = xbar, pbar_D = G xbar.
+//
+// Activations are checkpointed once per outer iteration (x_k and y1'); the
+// inner TV states are recomputed in the backward pass (forward updates with
+// every p_u kept for that one outer iteration).  Layout (T,H,W) per sample.
+
+#include <cmath>
+
+#include "scigap_internal.h"
+
+namespace scigap {
+
+namespace {
+
+constexpr int BX = 64, BYR = 4;
+constexpr float kTau = 0.25f;
+
+struct Plane3 {
+  int H, W, T;
+};
+
+__device__ __forceinline__ int2 pix() {
+  return make_int2(blockIdx.x * BX + int(threadIdx.x % BX), blockIdx.y * BYR + int(threadIdx.x / BX));
+}
+
+// G^T q at (r,c) for a dual field q (two planes q0, q1 of one frame).
+__device__ __forceinline__ float gt(const float* q0, const float* q1, int r, int c, int H, int W) {
+  const size_t o = size_t(r) * W + c;
+  float d = 0.f;
+  if (q0) {
+    const float up = r > 0 ? q0[o - W] : 0.f;
+    const float me = r < H - 1 ? q0[o] : 0.f;
+    const float lf = c > 0 ? q1[o - 1] : 0.f;
+    const float m1 = c < W - 1 ? q1[o] : 0.f;
+    d = (up - me) + (lf - m1);
+  }
+  return d;
+}
+
+// Block reduction of one float into a double accumulator (scaled).
+__device__ __forceinline__ void block_atomic(float v, double* dst, double scale) {
+  const float s = wave_sum(v);
+  __shared__ float part[BX * BYR / 64];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int w = 0; w < BX * BYR / 64; ++w) tot += part[w];
+    atomicAdd(dst, tot * scale);
+  }
+}
+
+// v = x_k + lambda * r * Phi with r = (y1' - yb)/S, yb = sum_t Phi_t x_k.
+__global__ void __launch_bounds__(256) k_ub_prep(const float* __restrict__ x,
+                                                 const float* __restrict__ y1p,
+                                                 const float* __restrict__ phi,
+                                                 const float* __restrict__ S, float lam,
+                                                 float* __restrict__ v, float* __restrict__ r_out,
+                                                 int64_t N, int T) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    float yb = 0.f;
+    for (int t = 0; t < T; ++t) yb = fmaf(phi[t * N + i], x[t * N + i], yb);
+    const float r = (y1p[i] - yb) / S[i];
+    r_out[i] = r;
+    const float lr = lam * r;
+    for (int t = 0; t < T; ++t) v[t * N + i] = fmaf(lr, phi[t * N + i], x[t * N + i]);
+  }
+}
+
+// One forward Chambolle update keeping p_{u+1} (p_in null for u = 0).
+__global__ void __launch_bounds__(256) k_ub_tvf(const float* __restrict__ v,
+                                                const float* __restrict__ p_in,
+                                                float* __restrict__ p_out, float tw, Plane3 g) {
+  const int2 q = pix();
+  const int r = q.y, c = q.x, t = blockIdx.z;
+  if (r >= g.H || c >= g.W) return;
+  const size_t N = size_t(g.H) * g.W, o = size_t(r) * g.W + c;
+  const float* vt = v + t * N;
+  const float* a0 = p_in ? p_in + size_t(t) * 2 * N : nullptr;
+  const float* a1 = a0 ? a0 + N : nullptr;
+  const float out = vt[o] + gt(a0, a1, r, c, g.H, g.W);
+  const float g0 = r < g.H - 1 ? (vt[o + g.W] + gt(a0, a1, r + 1, c, g.H, g.W)) - out : 0.f;
+  const float g1 = c < g.W - 1 ? (vt[o + 1] + gt(a0, a1, r, c + 1, g.H, g.W)) - out : 0.f;
+  const float n = fast_sqrt(fmaf(g0, g0, g1 * g1));
+  const float inv = fast_rcp(fmaf(tw, n, 1.f));
+  float* b0 = p_out + size_t(t) * 2 * N;
+  b0[o] = fmaf(-kTau, g0, a0 ? a0[o] : 0.f) * inv;
+  b0[N + o] = fmaf(-kTau, g1, a1 ? a1[o] : 0.f) * inv;
+}
+
+// Reverse of update u, part A: from pbar_{u+1} = abar_{u+1} + G obar_{u+1}
+// (abar_next null for the last update) compute abar_u and gbar_u, and add
+// the weight adjoint (dc/dw = -tau/w^2) to *wbar.
+__global__ void __launch_bounds__(256) k_ub_tvb_a(const float* __restrict__ v,
+                                                  const float* __restrict__ p_u,
+                                                  const float* __restrict__ abar_next,
+                                                  const float* __restrict__ obar_next,
+                                                  float* __restrict__ abar,
+                                                  float* __restrict__ gbar, float tw, float w,
+                                                  double* wbar, Plane3 g) {
+  const int2 q = pix();
+  const int r = q.y, c = q.x, t = blockIdx.z;
+  const bool in = r < g.H && c < g.W;
+  float cb = 0.f;
+  if (in) {
+    const size_t N = size_t(g.H) * g.W, o = size_t(r) * g.W + c;
+    const float* vt = v + t * N;
+    const float* a0 = p_u ? p_u + size_t(t) * 2 * N : nullptr;
+    const float* a1 = a0 ? a0 + N : nullptr;
+    const float out = vt[o] + gt(a0, a1, r, c, g.H, g.W);
+    const float g0 = r < g.H - 1 ? (vt[o + g.W] + gt(a0, a1, r + 1, c, g.H, g.W)) - out : 0.f;
+    const float g1 = c < g.W - 1 ? (vt[o + 1] + gt(a0, a1, r, c + 1, g.H, g.W)) - out : 0.f;
+    const float n = fast_sqrt(fmaf(g0, g0, g1 * g1));
+    const float ct = tw;  // c = tau / w
+    const float s = fast_rcp(fmaf(ct, n, 1.f));
+    const float pa0 = fmaf(-kTau, g0, a0 ? a0[o] : 0.f);
+    const float pa1 = fmaf(-kTau, g1, a1 ? a1[o] : 0.f);
+    // pbar_{u+1} = abar_{u+1} + G obar_{u+1}
+    const float* ob = obar_next + t * N;
+    float pb0 = r < g.H - 1 ? ob[o + g.W] - ob[o] : 0.f;
+    float pb1 = c < g.W - 1 ? ob[o + 1] - ob[o] : 0.f;
+    if (abar_next) {
+      const float* an = abar_next + size_t(t) * 2 * N;
+      pb0 += an[o];
+      pb1 += an[N + o];
+    }
+    const float ab0 = pb0 * s, ab1 = pb1 * s;
+    const float sbar = fmaf(pb0, pa0, pb1 * pa1);
+    const float nbar = -ct * s * s * sbar;
+    cb = -n * s * s * sbar;
+    const float k = n > 0.f ? nbar / n : 0.f;
+    float* ao = abar + size_t(t) * 2 * N;
+    ao[o] = ab0;
+    ao[N + o] = ab1;
+    float* go = gbar + size_t(t) * 2 * N;
+    go[o] = fmaf(k, g0, -kTau * ab0);
+    go[N + o] = fmaf(k, g1, -kTau * ab1);
+  }
+  block_atomic(cb, wbar, -double(kTau) / (double(w) * double(w)));
+}
+
+// Reverse of update u, part B: obar_u = G^T gbar_u; vbar += obar_u
+// (vbar = xbar + obar on the first reverse step, when `first`).
+__global__ void __launch_bounds__(256) k_ub_tvb_b(const float* __restrict__ gbar,
+                                                  float* __restrict__ obar,
+                                                  const float* __restrict__ xbar,
+                                                  float* __restrict__ vbar, int first, Plane3 g) {
+  const int2 q = pix();
+  const int r = q.y, c = q.x, t = blockIdx.z;
+  if (r >= g.H || c >= g.W) return;
+  const size_t N = size_t(g.H) * g.W, o = size_t(r) * g.W + c;
+  const float* g0 = gbar + size_t(t) * 2 * N;
+  const float ov = gt(g0, g0 + N, r, c, g.H, g.W);
+  obar[t * N + o] = ov;
+  vbar[t * N + o] = (first ? xbar[t * N + o] : vbar[t * N + o]) + ov;
+}
+
+// Reverse of the projection v = x + lambda*r*Phi, r = (y1' - yb)/S,
+// y1' = y1 + y - yb, yb = sum_t Phi_t x_t (per pixel, all frames).
+__global__ void __launch_bounds__(256) k_ub_proj(
+    const float* __restrict__ vbar, const float* __restrict__ x, const float* __restrict__ phi,
+    const float* __restrict__ S, const float* __restrict__ r, float lam,
+    const float* __restrict__ y1bar_ext, float* __restrict__ xbar_out,
+    float* __restrict__ y1bar_out, float* __restrict__ ybar, float* __restrict__ phibar,
+    double* lambar, int64_t N, int T) {
+  float lsum = 0.f;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const float ri = r[i], si = S[i];
+    float rbar = 0.f, fsum = 0.f;
+    for (int t = 0; t < T; ++t) {
+      const float vb = vbar[t * N + i], f = phi[t * N + i];
+      rbar = fmaf(vb, f, rbar);
+      fsum += f;
+    }
+    lsum = fmaf(rbar, ri, lsum);  // dL/dlambda = sum_t vbar*r*Phi
+    rbar *= lam;
+    const float y1tot = (y1bar_ext ? y1bar_ext[i] : 0.f) + rbar / si;
+    const float ybb = -rbar / si - y1tot;
+    // S = sum_t Phi_t, except where that sum is 0 and S is the constant 1.
+    const float sbar = fsum != 0.f ? -rbar * ri / si : 0.f;
+    y1bar_out[i] = y1tot;
+    ybar[i] += y1tot;
+    const float lr = lam * ri;
+    for (int t = 0; t < T; ++t) {
+      const size_t o = size_t(t) * N + i;
+      const float vb = vbar[o], f = phi[o], xv = x[o];
+      xbar_out[o] = fmaf(ybb, f, vb);
+      if (phibar) phibar[o] += fmaf(lr, vb, fmaf(ybb, xv, sbar));
+    }
+  }
+  block_atomic(lsum, lambar, 1.0);
+}
+
+// Reverse of x_0 = Phi*y: phibar += xbar_0 * y, ybar += sum_t xbar_0 * Phi.
+__global__ void __launch_bounds__(256) k_ub_init(const float* __restrict__ xbar,
+                                                 const float* __restrict__ phi,
+                                                 const float* __restrict__ y,
+                                                 float* __restrict__ ybar,
+                                                 float* __restrict__ phibar, int64_t N, int T) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    float acc = 0.f;
+    const float yv = y[i];
+    for (int t = 0; t < T; ++t) {
+      const size_t o = size_t(t) * N + i;
+      acc = fmaf(xbar[o], phi[o], acc);
+      if (phibar) phibar[o] = fmaf(xbar[o], yv, phibar[o]);
+    }
+    ybar[i] += acc;
+  }
+}
+
+__global__ void k_fill_f32(float* p, int64_t n, float v) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+unsigned grid1(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  return unsigned(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
+// phi = sigmoid(logits).
+__global__ void k_sigmoid(const float* __restrict__ z, float* __restrict__ phi, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    phi[i] = 1.f / (1.f + expf(-z[i]));
+}
+
+// dL/dlogits = dL/dphi * phi * (1 - phi).
+__global__ void k_sigmoid_bwd(const float* __restrict__ phibar, const float* __restrict__ phi,
+                              float* __restrict__ zbar, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const float f = phi[i];
+    zbar[i] = phibar[i] * f * (1.f - f);
+  }
+}
+
+// Reverse of y = sum_t phi_t * truth_t: phibar_t += ybar * truth_t.
+__global__ void k_measure_bwd(const float* __restrict__ ybar, const float* __restrict__ truth,
+                              float* __restrict__ phibar, int64_t N, int T) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const float yb = ybar[i];
+    for (int t = 0; t < T; ++t) phibar[size_t(t) * N + i] = fmaf(yb, truth[size_t(t) * N + i],
+                                                                 phibar[size_t(t) * N + i]);
+  }
+}
+
+}  // namespace
+
+size_t unrolled_backward_workspace(int64_t H, int64_t W, int64_t T, int D) {
+  const size_t N = size_t(H) * W, V = N * size_t(T);
+  // v, r, p_1..p_D, abar x2, gbar, obar x2, vbar, xbar x2, y1bar x2
+  return align256(V * 4) + align256(N * 4) + size_t(D) * align256(2 * V * 4) +
+         2 * align256(2 * V * 4) + align256(2 * V * 4) + 2 * align256(V * 4) + align256(V * 4) +
+         2 * align256(V * 4) + 2 * align256(N * 4);
+}
+
+// Backward through K unrolled iterations of one sample.
+int unrolled_backward(int64_t H, int64_t W, int64_t T, int K, int D, const float* lambdas,
+                      const float* weights, const float* y, const float* phi, const float* S,
+                      const float* ckpt_x, size_t xstride, const float* ckpt_y1,
+                      size_t y1stride, const float* grad_x,
+                      float* grad_phi, float* grad_y, double* grad_lambda, double* grad_weight,
+                      void* ws, hipStream_t s) {
+  const int64_t N = H * W, V = N * T;
+  char* base = static_cast<char*>(ws);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* q = base + off;
+    off += align256(bytes);
+    return reinterpret_cast<float*>(q);
+  };
+  float* v = take(V * 4);
+  float* r = take(N * 4);
+  float* P[9] = {nullptr};
+  for (int u = 1; u <= D; ++u) P[u] = take(2 * V * 4);
+  float* abar[2] = {take(2 * V * 4), take(2 * V * 4)};
+  float* gbar = take(2 * V * 4);
+  float* obar[2] = {take(V * 4), take(V * 4)};
+  float* vbar = take(V * 4);
+  float* xbar[2] = {take(V * 4), take(V * 4)};
+  float* y1bar[2] = {take(N * 4), take(N * 4)};
+  const Plane3 g{int(H), int(W), int(T)};
+  const dim3 sgrid(unsigned((W + BX - 1) / BX), unsigned((H + BYR - 1) / BYR), unsigned(T));
+
+  k_fill_f32<<<grid1(N), 256, 0, s>>>(grad_y, N, 0.f);
+  SCIGAP_LAUNCH_CHECK("grad fill");
+  const float* xb_in = grad_x;  // adjoint of x_{k+1}
+  const float* y1b_in = nullptr;
+  int xsel = 0, ysel = 0;
+  for (int k = K - 1; k >= 0; --k) {
+    const float lam = lambdas[k], w = weights[k], tw = float(0.25 / double(w));
+    const float* xk = ckpt_x + size_t(k) * xstride;
+    const float* y1p = ckpt_y1 + size_t(k) * y1stride;  // y1' of iteration k
+    {
+      ProfScope ps("unrolled_bwd_recompute", s);
+      k_ub_prep<<<grid1(N), 256, 0, s>>>(xk, y1p, phi, S, lam, v, r, N, int(T));
+      for (int u = 0; u < D; ++u)
+        k_ub_tvf<<<sgrid, BX * BYR, 0, s>>>(v, u == 0 ? nullptr : P[u], P[u + 1], tw, g);
+    }
+    SCIGAP_LAUNCH_CHECK("unrolled recompute");
+    {
+      ProfScope ps("unrolled_bwd_tv", s);
+      for (int u = D - 1; u >= 0; --u) {
+        const int cur = u & 1, nxt = (u + 1) & 1;
+        const float* an = u == D - 1 ? nullptr : abar[nxt];
+        const float* on = u == D - 1 ? xb_in : obar[nxt];
+        k_ub_tvb_a<<<sgrid, BX * BYR, 0, s>>>(v, u == 0 ? nullptr : P[u], an, on, abar[cur], gbar,
+                                               tw, w, grad_weight + k, g);
+        k_ub_tvb_b<<<sgrid, BX * BYR, 0, s>>>(gbar, obar[cur], xb_in, vbar, u == D - 1 ? 1 : 0, g);
+      }
+    }
+    SCIGAP_LAUNCH_CHECK("unrolled tv backward");
+    {
+      ProfScope ps("unrolled_bwd_proj", s);
+      k_ub_proj<<<grid1(N), 256, 0, s>>>(vbar, xk, phi, S, r, lam, y1b_in, xbar[xsel],
+                                         y1bar[ysel], grad_y, grad_phi, grad_lambda + k, N,
+                                         int(T));
+    }
+    SCIGAP_LAUNCH_CHECK("unrolled projection backward");
+    xb_in = xbar[xsel];
+    y1b_in = y1bar[ysel];
+    xsel ^= 1;
+    ysel ^= 1;
+  }
+  // x_0 = Phi * y
+  k_ub_init<<<grid1(N), 256, 0, s>>>(xb_in, phi, y, grad_y, grad_phi, N, int(T));
+  SCIGAP_LAUNCH_CHECK("unrolled init backward");
+  return SCIGAP_OK;
+}
+
+}  // namespace scigap
+
+// ---------------------------------------------------------------------------
+// C entry points
+// ---------------------------------------------------------------------------
+
+namespace scigap {
+
+int fill_f64(double* p, int n, double v, hipStream_t s);
+int unrolled_forward_iterations(int64_t H, int64_t W, int64_t T, int K, int D,
+                                const float* lambdas, const float* weights, const float* y,
+                                const float* phi, const float* S, float* xs, size_t xstride,
+                                float* y1s, size_t y1stride, float* x_out, float* r2,
+                                double* energy, int* stop, int* nexit, hipStream_t s);
+
+namespace {
+
+// Checkpoint of one sample: x_0..x_{K-1}, y1'_0..y1'_{K-1}, y; every plane
+// set starts on a 256-byte boundary (strides in floats).
+struct CkptLayout {
+  size_t x = 0, y1 = 0, y = 0, per_sample = 0, total = 0, xstride = 0, y1stride = 0;
+};
+
+CkptLayout ckpt_layout(const scigap_unrolled& u) {
+  const size_t N = size_t(u.H) * u.W, V = N * size_t(u.T), K = size_t(u.iters);
+  CkptLayout c;
+  c.xstride = align256(V * 4) / 4;
+  c.y1stride = align256(N * 4) / 4;
+  c.x = 0;
+  c.y1 = K * c.xstride * 4;
+  c.y = c.y1 + K * c.y1stride * 4;
+  c.per_sample = c.y + align256(N * 4);
+  c.total = c.per_sample * size_t(u.batch);
+  return c;
+}
+
+struct UWorkspace {
+  float* phi = nullptr;     // sigmoid(logits), when logits
+  float* S = nullptr;       // phi_sum
+  float* r2 = nullptr;      // two residual planes
+  double* energy = nullptr; // K*T*D
+  int* stop = nullptr;      // T
+  int* nexit = nullptr;     // K
+  float* phibar = nullptr;  // (T,H,W) dL/dphi, when logits
+  float* ybar = nullptr;    // (H,W) dL/dy of one sample
+  double* gl = nullptr;     // K, when the caller passes no grad_lambda
+  double* gw = nullptr;     // K, when the caller passes no grad_weight
+  void* bwd = nullptr;
+  size_t bytes = 0;
+};
+
+UWorkspace carve(const scigap_unrolled& u, char* base) {
+  const size_t N = size_t(u.H) * u.W, V = N * size_t(u.T), K = size_t(u.iters);
+  const int D = u.tv_iter_max - 1;
+  UWorkspace w;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* q = base ? base + off : nullptr;
+    off += align256(bytes);
+    return q;
+  };
+  if (u.logits) w.phi = reinterpret_cast<float*>(take(V * 4));
+  w.S = reinterpret_cast<float*>(take(N * 4));
+  w.r2 = reinterpret_cast<float*>(take(2 * N * 4));
+  w.energy = reinterpret_cast<double*>(take(K * size_t(u.T) * size_t(D) * 8));
+  w.stop = reinterpret_cast<int*>(take(size_t(u.T) * 4));
+  w.nexit = reinterpret_cast<int*>(take(K * 4));
+  if (u.logits) w.phibar = reinterpret_cast<float*>(take(V * 4));
+  w.ybar = reinterpret_cast<float*>(take(N * 4));
+  w.gl = reinterpret_cast<double*>(take(K * 8));
+  w.gw = reinterpret_cast<double*>(take(K * 8));
+  w.bwd = take(unrolled_backward_workspace(u.H, u.W, u.T, D));
+  w.bytes = off;
+  return w;
+}
+
+int check_unrolled(const scigap_unrolled* u) {
+  if (!u) return fail(SCIGAP_ENULL, "unrolled descriptor is NULL");
+  if (int rc = check_shape(u->H, u->W, u->T)) return rc;
+  if (u->H * u->W >= (int64_t(1) << 30))
+    return fail(SCIGAP_EINVAL, "H*W must be < 2^30 (buffer-addressed planes)");
+  if (u->batch < 1) return fail(SCIGAP_EINVAL, "batch must be >= 1");
+  if (u->iters < 1) return fail(SCIGAP_EINVAL, "iters must be >= 1");
+  if (u->tv_iter_max < 2 || u->tv_iter_max > 7)
+    return fail(SCIGAP_EUNSUPPORTED, "tv_iter_max must be in [2, 7] (fused TV depth)");
+  if (u->batch > 1 && (u->H * u->W * u->T) % 2 != 0)
+    return fail(SCIGAP_EINVAL, "batch > 1 needs an even T*H*W (8-byte aligned samples)");
+  return SCIGAP_OK;
+}
+
+int check_common(const scigap_unrolled* u, const float* lambdas, const float* weights,
+                 const float* mask, const float* meas, const void* ckpt, size_t ckpt_bytes,
+                 void* ws, size_t ws_bytes) {
+  if (int rc = check_unrolled(u)) return rc;
+  if (!lambdas || !weights || !mask || !meas || !ckpt)
+    return fail(SCIGAP_ENULL, "unrolled: null argument");
+  for (int k = 0; k < u->iters; ++k) {
+    if (!(weights[k] > 0.f) || !std::isfinite(weights[k]))
+      return fail(SCIGAP_EINVAL, "tv weight %d must be finite and > 0", k);
+    if (!std::isfinite(lambdas[k])) return fail(SCIGAP_EINVAL, "lambda %d is not finite", k);
+  }
+  const size_t need = ckpt_layout(*u).total;
+  if (ckpt_bytes < need)
+    return fail(SCIGAP_EWORKSPACE, "checkpoint needs %zu bytes, got %zu", need, ckpt_bytes);
+  const size_t wneed = carve(*u, nullptr).bytes;
+  if (!ws || ws_bytes < wneed)
+    return fail(SCIGAP_EWORKSPACE, "workspace needs %zu bytes, got %zu", wneed, ws_bytes);
+  if ((reinterpret_cast<uintptr_t>(mask) | reinterpret_cast<uintptr_t>(ckpt)) % 8 != 0)
+    return fail(SCIGAP_EINVAL, "device buffers must be 8-byte aligned");
+  return SCIGAP_OK;
+}
+
+// phi (from logits when asked) and its frame sum.
+int prepare_mask(const scigap_unrolled& u, const float* mask, const UWorkspace& w,
+                 const float** phi, hipStream_t s) {
+  const int64_t N = u.H * u.W, V = N * u.T;
+  *phi = mask;
+  if (u.logits) {
+    k_sigmoid<<<grid1(V), 256, 0, s>>>(mask, w.phi, V);
+    SCIGAP_LAUNCH_CHECK("sigmoid launch");
+    *phi = w.phi;
+  }
+  return scigap_phi_sum(*phi, w.S, u.H, u.W, u.T, s);
+}
+
+}  // namespace
+}  // namespace scigap
+
+using namespace scigap;
+
+extern "C" {
+
+size_t scigap_unrolled_checkpoint_bytes(const scigap_unrolled* u) {
+  if (check_unrolled(u) != SCIGAP_OK) return 0;
+  return ckpt_layout(*u).total;
+}
+
+size_t scigap_unrolled_workspace_bytes(const scigap_unrolled* u) {
+  if (check_unrolled(u) != SCIGAP_OK) return 0;
+  return carve(*u, nullptr).bytes;
+}
+
+int scigap_unrolled_forward(const scigap_unrolled* u, const float* lambdas, const float* weights,
+                            const float* mask, const float* meas, float* x_out, void* ckpt,
+                            size_t ckpt_bytes, void* workspace, size_t workspace_bytes,
+                            void* stream) {
+  clear_error();
+  if (int rc = check_common(u, lambdas, weights, mask, meas, ckpt, ckpt_bytes, workspace,
+                            workspace_bytes))
+    return rc;
+  if (!x_out) return fail(SCIGAP_ENULL, "unrolled: null x_out");
+  if (reinterpret_cast<uintptr_t>(x_out) % 8 != 0)
+    return fail(SCIGAP_EINVAL, "device buffers must be 8-byte aligned");
+  const int64_t N = u->H * u->W, V = N * u->T;
+  const size_t outb = size_t(V) * 4 * u->batch;
+  const CkptLayout cl = ckpt_layout(*u);
+  if (overlaps(x_out, outb, ckpt, cl.total) || overlaps(x_out, outb, workspace, workspace_bytes))
+    return fail(SCIGAP_EALIAS, "x_out overlaps the checkpoint or the workspace");
+  hipStream_t s = as_stream(stream);
+  const UWorkspace w = carve(*u, static_cast<char*>(workspace));
+  const float* phi = nullptr;
+  if (int rc = prepare_mask(*u, mask, w, &phi, s)) return rc;
+  for (int b = 0; b < u->batch; ++b) {
+    char* cb = static_cast<char*>(ckpt) + cl.per_sample * b;
+    float* xs = reinterpret_cast<float*>(cb + cl.x);
+    float* y1s = reinterpret_cast<float*>(cb + cl.y1);
+    float* yb = reinterpret_cast<float*>(cb + cl.y);
+    if (u->measure) {
+      if (int rc = scigap_forward(meas + size_t(b) * V, phi, yb, u->H, u->W, u->T, s)) return rc;
+    } else {
+      hipError_t e = hipMemcpyAsync(yb, meas + size_t(b) * N, size_t(N) * 4,
+                                    hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) return hip_fail(e, "copy y");
+    }
+    if (int rc = scigap_adjoint(yb, phi, xs, u->H, u->W, u->T, s)) return rc;  // x_0 = At(y)
+    if (int rc = unrolled_forward_iterations(u->H, u->W, u->T, u->iters, u->tv_iter_max - 1,
+                                             lambdas, weights, yb, phi, w.S, xs, cl.xstride,
+                                             y1s, cl.y1stride, x_out + size_t(b) * V, w.r2, w.energy, w.stop,
+                                             w.nexit, s))
+      return rc;
+  }
+  return SCIGAP_OK;
+}
+
+int scigap_unrolled_backward(const scigap_unrolled* u, const float* lambdas, const float* weights,
+                             const float* mask, const float* meas, const void* ckpt,
+                             size_t ckpt_bytes, const float* grad_x, float* grad_mask,
+                             float* grad_meas, double* grad_lambda, double* grad_weight,
+                             void* workspace, size_t workspace_bytes, void* stream) {
+  clear_error();
+  if (int rc = check_common(u, lambdas, weights, mask, meas, ckpt, ckpt_bytes, workspace,
+                            workspace_bytes))
+    return rc;
+  if (!grad_x) return fail(SCIGAP_ENULL, "unrolled: null grad_x");
+  if (u->measure && grad_meas)
+    return fail(SCIGAP_EINVAL, "measure = 1: the truth videos get no gradient (grad_meas NULL)");
+  const int64_t N = u->H * u->W, V = N * u->T;
+  const int K = u->iters, D = u->tv_iter_max - 1;
+  const CkptLayout cl = ckpt_layout(*u);
+  if ((grad_mask && (overlaps(grad_mask, V * 4, workspace, workspace_bytes) ||
+                     overlaps(grad_mask, V * 4, ckpt, cl.total))) ||
+      (grad_meas && overlaps(grad_meas, size_t(N) * 4 * u->batch, workspace, workspace_bytes)))
+    return fail(SCIGAP_EALIAS, "a gradient output overlaps the checkpoint or the workspace");
+  hipStream_t s = as_stream(stream);
+  const UWorkspace w = carve(*u, static_cast<char*>(workspace));
+  const float* phi = nullptr;
+  if (int rc = prepare_mask(*u, mask, w, &phi, s)) return rc;
+  double* gl = grad_lambda ? grad_lambda : w.gl;
+  double* gw = grad_weight ? grad_weight : w.gw;
+  if (int rc = fill_f64(gl, K, 0.0, s)) return rc;
+  if (int rc = fill_f64(gw, K, 0.0, s)) return rc;
+  float* phibar = grad_mask ? (u->logits ? w.phibar : grad_mask) : nullptr;
+  if (phibar) {
+    k_fill_f32<<<grid1(V), 256, 0, s>>>(phibar, V, 0.f);
+    SCIGAP_LAUNCH_CHECK("gradient fill");
+  }
+  for (int b = 0; b < u->batch; ++b) {
+    const char* cb = static_cast<const char*>(ckpt) + cl.per_sample * b;
+    const float* xs = reinterpret_cast<const float*>(cb + cl.x);
+    const float* y1s = reinterpret_cast<const float*>(cb + cl.y1);
+    const float* yb = reinterpret_cast<const float*>(cb + cl.y);
+    if (int rc = unrolled_backward(u->H, u->W, u->T, K, D, lambdas, weights, yb, phi, w.S, xs,
+                                   cl.xstride, y1s, cl.y1stride, grad_x + size_t(b) * V, phibar, w.ybar, gl, gw, w.bwd,
+                                   s))
+      return rc;
+    if (u->measure && phibar) {
+      k_measure_bwd<<<grid1(N), 256, 0, s>>>(w.ybar, meas + size_t(b) * V, phibar, N, int(u->T));
+      SCIGAP_LAUNCH_CHECK("measurement backward launch");
+    }
+    if (!u->measure && grad_meas) {
+      hipError_t e = hipMemcpyAsync(grad_meas + size_t(b) * N, w.ybar, size_t(N) * 4,
+                                    hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) return hip_fail(e, "copy dL/dy");
+    }
+  }
+  if (grad_mask && u->logits) {
+    k_sigmoid_bwd<<<grid1(V), 256, 0, s>>>(w.phibar, phi, grad_mask, V);
+    SCIGAP_LAUNCH_CHECK("sigmoid backward launch");
+  }
+  return SCIGAP_OK;
+}
+
+}  // extern "C"

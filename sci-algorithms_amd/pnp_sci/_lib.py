@@ -14,7 +14,7 @@ import threading
 from .build import LIB_PATH
 
 __all__ = ["ScigapError", "ScigapUnavailable", "Params", "KernelStat", "BatchItem",
-           "BatchOptions", "lib", "check",
+           "BatchOptions", "Unrolled", "lib", "check",
            "stream_ptr", "kernel_profile", "SYMBOLS", "METHOD_GAP", "METHOD_ADMM"]
 
 METHOD_GAP = 0
@@ -67,6 +67,14 @@ class BatchOptions(ctypes.Structure):
                 ("order", ctypes.c_int)]
 
 
+class Unrolled(ctypes.Structure):
+    """Mirror of ``scigap_unrolled``."""
+
+    _fields_ = [("H", ctypes.c_int64), ("W", ctypes.c_int64), ("T", ctypes.c_int64),
+                ("batch", ctypes.c_int), ("iters", ctypes.c_int), ("tv_iter_max", ctypes.c_int),
+                ("logits", ctypes.c_int), ("measure", ctypes.c_int)]
+
+
 class KernelStat(ctypes.Structure):
     """Mirror of ``scigap_kernel_stat``."""
 
@@ -104,6 +112,13 @@ SYMBOLS = {
                                         ctypes.POINTER(BatchOptions), _P]),
     "scigap_batch_run": (_INT, [_P, _P]),
     "scigap_batch_plan_destroy": (_INT, [_P]),
+    "scigap_unrolled_checkpoint_bytes": (ctypes.c_size_t, [ctypes.POINTER(Unrolled)]),
+    "scigap_unrolled_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(Unrolled)]),
+    "scigap_unrolled_forward": (_INT, [ctypes.POINTER(Unrolled), _P, _P, _P, _P, _P, _P,
+                                       ctypes.c_size_t, _P, ctypes.c_size_t, _P]),
+    "scigap_unrolled_backward": (_INT, [ctypes.POINTER(Unrolled), _P, _P, _P, _P, _P,
+                                        ctypes.c_size_t, _P, _P, _P, _P, _P, _P,
+                                        ctypes.c_size_t, _P]),
     "scigap_profile_begin": (_INT, []),
     "scigap_profile_end": (_INT, [ctypes.POINTER(KernelStat), _INT, ctypes.POINTER(_INT)]),
 }

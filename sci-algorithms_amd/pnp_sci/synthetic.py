@@ -130,3 +130,31 @@ def config3_sizes(seed: int = 0, count: int = 64):
 def config3_problem(k: int, size, p: float, seed: int = 0) -> Problem:
     H, W, T = size
     return make_problem(H, W, T, seed=seed * 1000 + k + 1, p=p)
+
+
+def unrolled_batch(batch: int, H: int, W: int, T: int, seed: int = 0, mask: str = "logits"):
+    """Training batch for the unrolled GAP-TV: truth videos (batch,T,H,W) and
+    either mask logits ~ N(0,1) (T,H,W) (``mask="logits"``, phi = sigmoid) or
+    a shifting binary Bernoulli(0.5) mask (``mask="shift"``), float32."""
+    rng = np.random.default_rng(seed)
+    truth = np.stack([truth_video(rng, H, W, T, frame_major=True) for _ in range(batch)])
+    if mask == "logits":
+        m = rng.standard_normal((T, H, W)).astype(np.float32)
+    elif mask == "shift":
+        base = (rng.random((H, W)) < 0.5).astype(np.float32)
+        m = np.stack([np.roll(base, t, axis=0) for t in range(T)], axis=0)
+    else:
+        raise ValueError("unknown mask kind %r" % mask)
+    return truth, m
+
+
+def config4(seed: int = 0):
+    """Unrolled GAP-TV training: batch 4 at 256x256x8, mask logits ~ N(0,1)
+    (K=10 outer x 5 TV iterations)."""
+    return unrolled_batch(4, 256, 256, 8, seed=seed, mask="logits")
+
+
+def config5(seed: int = 0):
+    """Large unrolled GAP-TV: batch 2 at 1024x1024x20 with a shifting binary
+    mask (K=20 outer x 5 TV iterations, learnable lambda_k / tv_weight_k)."""
+    return unrolled_batch(2, 1024, 1024, 20, seed=seed, mask="shift")

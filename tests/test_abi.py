@@ -79,3 +79,39 @@ def test_workspace_sizes_and_params(dll):
                           ctypes.c_void_p(1 << 23), None, ctypes.c_void_p(1 << 24), None, None,
                           None, None, 0, None)
     assert rc == -4 and b"workspace" in dll.scigap_last_error()
+
+
+def test_unrolled_sizes_and_validation(dll):
+    u = _lib.Unrolled(H=64, W=48, T=4, batch=2, iters=3, tv_iter_max=5, logits=1, measure=1)
+    N, V = 64 * 48, 64 * 48 * 4
+    ck = dll.scigap_unrolled_checkpoint_bytes(ctypes.byref(u))
+    assert ck >= 2 * (3 * V * 4 + 3 * N * 4 + N * 4)      # x_k, y1_k, y per sample
+    ws = dll.scigap_unrolled_workspace_bytes(ctypes.byref(u))
+    assert ws >= 4 * 2 * V * 4                            # the recomputed dual fields p_1..p_4
+    for field, bad in (("tv_iter_max", 8), ("tv_iter_max", 1), ("iters", 0), ("batch", 0)):
+        v = _lib.Unrolled.from_buffer_copy(u)
+        setattr(v, field, bad)
+        assert dll.scigap_unrolled_checkpoint_bytes(ctypes.byref(v)) == 0
+        assert dll.scigap_unrolled_workspace_bytes(ctypes.byref(v)) == 0
+    odd = _lib.Unrolled(H=5, W=7, T=3, batch=2, iters=2, tv_iter_max=3, logits=0, measure=0)
+    assert dll.scigap_unrolled_checkpoint_bytes(ctypes.byref(odd)) == 0
+    odd.batch = 1
+    assert dll.scigap_unrolled_checkpoint_bytes(ctypes.byref(odd)) > 0
+    # host-side checks run before any device work
+    lam = (ctypes.c_float * 3)(1, 1, 1)
+    w = (ctypes.c_float * 3)(0.1, 0.0, 0.1)
+    big = ctypes.c_void_p(1 << 30)
+    rc = dll.scigap_unrolled_forward(ctypes.byref(u), lam, w, ctypes.c_void_p(256),
+                                     ctypes.c_void_p(1 << 20), ctypes.c_void_p(1 << 24), big, ck,
+                                     ctypes.c_void_p(1 << 32), ws, None)
+    assert rc == -1 and b"weight" in dll.scigap_last_error()
+    w[1] = 0.1
+    rc = dll.scigap_unrolled_forward(ctypes.byref(u), lam, w, ctypes.c_void_p(256),
+                                     ctypes.c_void_p(1 << 20), ctypes.c_void_p(1 << 24), big,
+                                     ck - 1, ctypes.c_void_p(1 << 32), ws, None)
+    assert rc == -4 and b"checkpoint" in dll.scigap_last_error()
+    rc = dll.scigap_unrolled_backward(ctypes.byref(u), lam, w, ctypes.c_void_p(256),
+                                      ctypes.c_void_p(1 << 20), big, ck, ctypes.c_void_p(1 << 24),
+                                      None, ctypes.c_void_p(1 << 25), None, None,
+                                      ctypes.c_void_p(1 << 32), ws, None)
+    assert rc == -1 and b"truth" in dll.scigap_last_error()   # measure=1 gives no dL/dtruth

@@ -313,5 +313,8 @@ def test_identity_tv_projection_is_exact_at_full_size(gpu):
     _lib.check(gpu.scigap_forward(x.data_ptr(), phi.data_ptr(), yb.data_ptr(), H, W, T,
                                   _lib.stream_ptr()))
     seen = phi.sum(0) > 0
-    assert float((yb - y)[seen].abs().max()) < 5e-6
+    # A(x) - y is formed from sums of up to T terms of size <= max(y): a few
+    # fp32 roundings at the scale T*max(y) bound it.
+    tol = 8 * np.finfo(np.float32).eps * T * float(y.max())
+    assert float((yb - y)[seen].abs().max()) < tol
     assert float(yb[~seen].abs().max()) == 0.0
