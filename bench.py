@@ -1,24 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark the fused GAP-TV reconstruction on one MI355X.
 
-Workload (default): accelerated GAP-TV on 2048x2048x20, shifting Bernoulli(0.5)
+Headline workload: accelerated GAP-TV on 2048x2048x20, shifting Bernoulli(0.5)
 mask, Gaussian measurement noise sigma = 0.01*max(y), lambda=1, 60 outer
 iterations, tv_weight=0.05, 5 TV iterations per outer iteration, synthetic
 seeded data.  One step = one full reconstruction (60 outer iterations) with
 all inputs already resident in HBM.
 
-Prints one JSON line: metric/value in pixel*frame*iter/s (H*W*T*K / time),
-the roofline of the dominant kernel (fused outer iteration, interior tiles)
-from HIP-event timings taken in this run, the reconstruction quality (PSNR vs
-the synthetic truth, relative error vs the FP64 C oracle when it is built),
-and the reference algorithm's NumPy path timed on this host as cpu_baseline.
+Prints one JSON line: metric/value in pixel*frame*iter/s (H*W*T*K / time);
+the roofline of the dominant kernel (the fused outer iteration) from HIP-event
+timings taken in this run, with its HBM traffic from two rocprofv3 --pmc
+passes (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a plain 1 GiB
+elementwise copy; the batched mixed-size queue (64 solves) against the same
+solves run one at a time; the unrolled differentiable GAP-TV (forward +
+backward time, peak device memory); and a CPU leg that times the reference
+algorithm's NumPy restatement on this host and holds every FP64 check.
 """
 
 import argparse
 import json
 import math
 import os
+import re
+import shutil
+import sqlite3
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -35,21 +43,356 @@ CONFIGS = {
 }
 
 
-def cpu_baseline(prob, K_sample, tv_weight):
-    """The reference's NumPy path (the oracle's restatement, fp32, one thread)
-    on the first K_sample outer iterations of the same solve."""
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def gap_bytes_per_pixel(T):
+    """Algorithmic HBM bytes of one accelerated GAP-TV outer iteration per
+    pixel: read x, Phi (T planes each), y, y1, Phi_sum; write x (T), y1."""
+    return (3 * T + 4) * 4
+
+
+# ---------------------------------------------------------------------------
+# Headline: one full-size solve
+# ---------------------------------------------------------------------------
+
+class Solver:
+    """Pre-allocated single-solve state on the device."""
+
+    def __init__(self, y, phi, ps, iters, tv_weight, tile_rows=0):
+        import torch
+        from pnp_sci import _lib
+        self._lib = _lib
+        self.L = _lib.lib()
+        T, H, W = phi.shape
+        self.y, self.phi, self.ps = y, phi, ps
+        p = _lib.Params()
+        self.L.scigap_default_params(p, H, W, T)
+        p.iters = iters
+        p.tv_weight = tv_weight
+        p.tv_iter_max = 5
+        p.tile_rows = tile_rows
+        self.p = p
+        self.nbytes = self.L.scigap_solve_workspace_bytes(p)
+        self.ws = torch.empty(self.nbytes, dtype=torch.uint8, device="cuda")
+        self.x = torch.empty((T, H, W), dtype=torch.float32, device="cuda")
+        self.stats = torch.empty(max(iters, 1), dtype=torch.int32, device="cuda")
+        self.stream = _lib.stream_ptr()
+
+    def __call__(self):
+        self._lib.check(self.L.scigap_solve(self.p, self.y.data_ptr(), self.phi.data_ptr(),
+                                            self.ps.data_ptr(), None, self.x.data_ptr(), None,
+                                            None, self.stats.data_ptr(), self.ws.data_ptr(),
+                                            self.nbytes, self.stream))
+
+
+def headline(args, prob):
     import numpy as np
+    import torch
+    from pnp_sci import _lib
+    from pnp_sci.device import to_device
+    H, W, T, K, tvw, _, _ = CONFIGS[args.config]
+    solve = Solver(to_device(prob.y), to_device(prob.mask_thw), to_device(prob.phi_sum), K, tvw)
+    for _ in range(args.warmup):
+        solve()
+    torch.cuda.synchronize()
+    log("timing %d steps" % args.steps)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solve()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+
+    # Dominant kernel, timed with HIP events on the launch stream.
+    prof = _lib.kernel_profile(solve)
+    n_it, ms_it = prof.get("fused_iteration", (0, 0.0))
+    if n_it == 0:
+        raise RuntimeError("no fused_iteration launches were timed: %s" % prof)
+    bpp = gap_bytes_per_pixel(T)
+    avg_it = ms_it / n_it                      # one launch covers all H*W pixels
+    achieved = bpp * H * W / (avg_it * 1e-3) / 1e9
+    iter_ms = sum(v[1] for k, v in prof.items() if k.startswith("fused")) / n_it
+    iter_gbs = bpp * H * W / (iter_ms * 1e-3) / 1e9
+    x = solve.x.cpu().numpy()
+    mse = float(np.mean((x.astype(np.float64) - prob.truth_thw) ** 2))
+    res = {
+        "value": H * W * T * K / dt, "ms_per_step": dt * 1e3,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_fused (one outer iteration, all tiles)",
+                     "avg_launch_ms": avg_it,
+                     "algorithmic_bytes_per_launch": bpp * H * W,
+                     "algorithmic_bytes_per_pixel": bpp},
+        "outer_iteration": {"ms": iter_ms, "algorithmic_GBps": iter_gbs,
+                            "frac": iter_gbs / HBM_PEAK_GBS,
+                            "kernels": {k: {"launches": v[0], "avg_ms": v[1] / max(v[0], 1)}
+                                        for k, v in prof.items()}},
+        "quality": {"psnr_db": 10 * math.log10(1.0 / mse),
+                    "early_exit_frames": int(solve.stats[:K].sum().item())},
+    }
+    return res, solve
+
+
+# ---------------------------------------------------------------------------
+# HBM traffic of the fused kernel from rocprofv3 PMC counters
+# ---------------------------------------------------------------------------
+
+def traffic_child():
+    """Run under rocprofv3: a 1 GiB elementwise copy (calibration), then three
+    fused outer iterations on a 2048x2048x20 problem."""
+    import torch
+    from pnp_sci.device import to_device
+    from pnp_sci.synthetic import make_problem
+    src = torch.rand(1 << 28, device="cuda")
+    dst = torch.empty_like(src)
+    torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    prob = make_problem(2048, 2048, 20, seed=0, mask="shift", noise_rel=0.01)
+    solve = Solver(to_device(prob.y), to_device(prob.mask_thw), to_device(prob.phi_sum), 3, 0.05)
+    solve()
+    torch.cuda.synchronize()
+
+
+def _pmc_pass(counter, outdir):
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    cmd = [exe, "--pmc", counter, "-d", outdir, "-o", "run", "--", sys.executable,
+           os.path.abspath(__file__), "--traffic-child"]
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       timeout=180)
+    dbs = [os.path.join(d, f) for d, _, fs in os.walk(outdir) for f in fs
+           if f.endswith("_results.db")]
+    if not dbs:
+        raise RuntimeError("rocprofv3 --pmc %s wrote no database (rc %d): %s"
+                           % (counter, r.returncode, r.stdout[-400:].decode(errors="replace")))
+    rows = []
+    for db in dbs:
+        con = sqlite3.connect(db)
+        rows += con.execute("select kernel_name, value from counters_collection "
+                            "where counter_name = ?", (counter,)).fetchall()
+        con.close()
+    fused = [v for n, v in rows if re.search(r"k_fused<", n)]
+    # torch's elementwise neg kernel; failing a name match, the largest other
+    # kernel (the 1 GiB torch.rand fill writes as much as the copy does)
+    others = [(v, n) for n, v in rows if "scigap" not in n]
+    copy = [v for v, n in others if "neg" in n] or [max(others)[0]] if others else []
+    if not fused or not copy:
+        raise RuntimeError("counter rows for the fused kernel or the copy kernel are missing")
+    return sum(fused) / len(fused), max(copy)
+
+
+def measure_traffic():
+    """HBM bytes per fused launch: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch), each scaled by 1 GiB over what the counter reads for the
+    1 GiB copy."""
+    out = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            log("rocprofv3 --pmc %s" % counter)
+            fused_kib, copy_kib = _pmc_pass(counter, os.path.join(tmp, counter))
+            scale = (1 << 30) / (copy_kib * 1024.0)
+            out[counter] = {"fused_KiB": fused_kib, "copy_KiB_for_1GiB": copy_kib,
+                            "scale": scale, "bytes": fused_kib * 1024.0 * scale}
+    out["bytes"] = out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Batched mixed-size queue (64 solves) against one-at-a-time solves
+# ---------------------------------------------------------------------------
+
+def batched(args):
+    import numpy as np
+    import torch
+    from pnp_sci.batch import BatchPlan
+    from pnp_sci.device import to_device
+    from pnp_sci.synthetic import config3_problem, config3_sizes
+    K, tvw = 40, 0.1
+    log("generating the 64-solve mixed-size queue")
+    draws = config3_sizes(seed=0, count=args.batch_count)
+    probs = [config3_problem(k, size, p, seed=0) for k, (size, p) in enumerate(draws)]
+    dev = [(to_device(pr.y), to_device(pr.mask_thw), to_device(pr.phi_sum), None)
+           for pr in probs]
+    pfi = sum(pr.mask_thw.size for pr in probs) * K       # pixel*frame*iter of the queue
+    bytes_it = sum(gap_bytes_per_pixel(pr.mask_thw.shape[0]) * pr.y.size for pr in probs)
+
+    singles = [Solver(y, phi, ps, K, tvw) for (y, phi, ps, _) in dev]
+    for s in singles:
+        s()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in singles:
+        s()
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+
+    plan = BatchPlan(dev, iters=K, tv_weight=tvw, use_graph=True)
+    plan.run()
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.run()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+
+    def mean_psnr(xs):
+        v = []
+        for pr, x in zip(probs, xs):
+            mse = float(np.mean((x.cpu().numpy().astype(np.float64) - pr.truth_thw) ** 2))
+            v.append(10 * math.log10(1.0 / mse))
+        return float(np.mean(v))
+
+    ps_seq = mean_psnr([s.x for s in singles])
+    ps_batch = mean_psnr(plan.x)
+    plan.close()
+    sizes = {}
+    for (size, _p) in draws:
+        key = "%dx%dx%d" % size
+        sizes[key] = sizes.get(key, 0) + 1
+    return {
+        "workload": "%d GAP-TV solves, sizes drawn from {256x256x8, 512x512x10, 1024x1024x20, "
+                    "512x768x16}, Bernoulli(p~U(0.3,0.7)) masks, K=%d, tv_weight %g, 5 TV "
+                    "iterations" % (len(probs), K, tvw),
+        "size_counts": sizes,
+        "batched": {"value": pfi / t_batch, "ms": t_batch * 1e3,
+                    "algorithmic_GBps": bytes_it * K / t_batch / 1e9,
+                    "mean_psnr_db": ps_batch, "mode": "persistent tile queue, hipGraph replay"},
+        "one_at_a_time": {"value": pfi / t_seq, "ms": t_seq * 1e3,
+                          "mean_psnr_db": ps_seq,
+                          "mode": "scigap_solve per measurement, same fused kernels"},
+        "speedup": t_seq / t_batch, "unit": UNIT,
+    }
+
+
+# ---------------------------------------------------------------------------
+# Unrolled differentiable GAP-TV
+# ---------------------------------------------------------------------------
+
+def unrolled(args):
+    import torch
+    from pnp_sci.synthetic import config4, config5
+    from pnp_sci.unrolled import unrolled_gap_tv
+    out = {}
+    for name, (data, K, logits, w0) in (("batch4_256x256x8_K10", (config4, 10, True, 0.1)),
+                                        ("batch2_1024x1024x20_K20", (config5, 20, False, 0.05))):
+        truth, mask = data(seed=0)
+        tt = torch.tensor(truth, device="cuda")
+        m = torch.tensor(mask, device="cuda", requires_grad=logits)
+        lt = torch.full((K,), 1.0, device="cuda", requires_grad=True)
+        wt = torch.full((K,), w0, device="cuda", requires_grad=True)
+
+        x = unrolled_gap_tv(m, tt, lt, wt, logits=logits)   # warm-up
+        ((x - tt) ** 2).mean().backward()
+        del x
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        base = torch.cuda.memory_allocated()
+        reps = 3
+        fwd = bwd = 0.0
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            x = unrolled_gap_tv(m, tt, lt, wt, logits=logits)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            ((x - tt) ** 2).mean().backward()
+            torch.cuda.synchronize()
+            fwd += t1 - t0
+            bwd += time.perf_counter() - t1
+        fwd, bwd = fwd / reps, bwd / reps
+        total = fwd + bwd
+        peak = torch.cuda.max_memory_allocated() - base
+        B, T, H, W = truth.shape
+        out[name] = {"fwd_bwd_ms": total * 1e3, "fwd_ms": fwd * 1e3, "bwd_ms": bwd * 1e3,
+                     "peak_extra_GiB": peak / 2 ** 30, "batch": B, "shape": [T, H, W],
+                     "iters": K, "tv_iter_max": 5,
+                     "mask": "sigmoid(logits), learnable" if logits else "shifting binary, fixed",
+                     "pixel_frame_iter_per_s": B * T * H * W * K / total}
+        del x, tt, m, lt, wt
+        torch.cuda.empty_cache()
+    return out
+
+
+# ---------------------------------------------------------------------------
+# CPU leg: baseline timing of the reference algorithm and the FP64 checks
+# ---------------------------------------------------------------------------
+
+def cpu_leg(args, prob, solve, want_unrolled):
+    """Times the NumPy restatement of the reference path (fp32, one thread)
+    on a bounded sample of the headline solve, and holds every FP64 check of
+    this run: the C oracle on the full-size problem for the first
+    --check-iters outer iterations, and the FP64 torch reference of the
+    unrolled model (gradients of lambda_k / tv_weight_k)."""
+    import numpy as np
+    import torch
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pnp_sci_oracle as O
-    m = np.ascontiguousarray(prob.mask)  # the reference holds C-order (H,W,T) arrays
-    A = lambda v: O.A_(v, m)  # noqa: E731
-    At = lambda v: O.At_(v, m)  # noqa: E731
-    t0 = time.perf_counter()
-    O.gap_denoise(prob.y, prob.phi_sum, A, At, _lambda=1, accelerate=True, iter_max=K_sample,
-                  tv_weight=tv_weight, tv_iter_max=5, show_iqa=False)
-    dt = time.perf_counter() - t0
-    H, W, T = m.shape
-    return H * W * T * K_sample / dt, dt
+    from pnp_sci import _lib
+    from pnp_sci.device import thw_to_hwt
+    H, W, T, K, tvw, _, _ = CONFIGS[args.config]
+    checks = {}
+    if not args.no_check:
+        try:
+            import oracle_c
+            kc = min(args.check_iters, K)
+            log("FP64 C oracle, %d outer iterations" % kc)
+            short = Solver(solve.y, solve.phi, solve.ps, kc, tvw)
+            short()
+            t1 = time.perf_counter()
+            want = oracle_c.gap_tv(prob.y, prob.mask, prob.phi_sum, iters=kc, tv_weight=tvw,
+                                   tv_iter_max=5)
+            got = thw_to_hwt(short.x).cpu().numpy().astype(np.float64)
+            checks["rel_err_vs_fp64_oracle"] = float(np.linalg.norm(got - want)
+                                                     / np.linalg.norm(want))
+            checks["oracle_check"] = ("full %dx%dx%d problem, first %d outer iterations, FP64 C "
+                                      "oracle (%.0f s on the host)"
+                                      % (H, W, T, kc, time.perf_counter() - t1))
+        except ImportError as e:
+            checks["rel_err_vs_fp64_oracle"] = None
+            checks["oracle_check"] = str(e)
+    if want_unrolled:
+        import unrolled_ref as R
+        from pnp_sci.synthetic import config5
+        from pnp_sci.unrolled import unrolled_gap_tv
+        log("FP64 reference gradients of the 1024x1024x20 unrolled model")
+        truth, mask = config5(seed=0)
+        Ku = 20
+        tt = torch.tensor(truth, device="cuda")
+        lt = torch.full((Ku,), 1.0, device="cuda", requires_grad=True)
+        wt = torch.full((Ku,), 0.05, device="cuda", requires_grad=True)
+        x = unrolled_gap_tv(torch.tensor(mask, device="cuda"), tt, lt, wt, logits=False)
+        ((x - tt) ** 2).mean().backward()
+        del x
+        dd = dict(device="cuda", dtype=torch.float64)
+        l64 = torch.full((Ku,), 1.0, **dd, requires_grad=True)
+        w64 = torch.full((Ku,), 0.05, **dd, requires_grad=True)
+        R.mse_loss(torch.tensor(mask, **dd), tt.double(), l64, w64, checkpoint=True).backward()
+
+        def rel(a, b):
+            return float((a.double() - b).norm() / b.norm())
+
+        checks["unrolled_grad_rel_err_vs_fp64"] = {
+            "lambda": rel(lt.grad, l64.grad), "tv_weight": rel(wt.grad, w64.grad),
+            "reference": "plain PyTorch float64 model with per-iteration checkpoints"}
+        del tt, l64, w64
+        torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (%d outer iterations)" % args.cpu_iters)
+        m = np.ascontiguousarray(prob.mask)  # the reference holds C-order (H,W,T) arrays
+        t0 = time.perf_counter()
+        O.gap_denoise(prob.y, prob.phi_sum, lambda v: O.A_(v, m), lambda v: O.At_(v, m),
+                      _lambda=1, accelerate=True, iter_max=args.cpu_iters, tv_weight=tvw,
+                      tv_iter_max=5, show_iqa=False)
+        secs = time.perf_counter() - t0
+        cpu = {"value": H * W * T * args.cpu_iters / secs, "unit": UNIT, "cores": 1,
+               "kind": "port",
+               "sample": "first %d of %d outer iterations of the same %dx%dx%d solve, NumPy "
+                         "fp32 restatement of the reference path, one thread (%.1f s); host has "
+                         "%d CPUs" % (args.cpu_iters, K, H, W, T, secs, os.cpu_count() or 0)}
+    return cpu, checks
 
 
 def main():
@@ -61,132 +404,64 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=2,
                     help="outer iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-check", action="store_true", help="skip the FP64 oracle comparison")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 oracle comparisons")
     ap.add_argument("--check-iters", type=int, default=10,
                     help="outer iterations of the full-size FP64 oracle comparison")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 PMC passes")
+    ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch")
+    ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--no-unrolled", action="store_true", help="skip the unrolled model")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the GAP-TV path is built and measured on a single "
               "MI355X (sharding across GPUs is design-only); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.traffic_child:
+        traffic_child()
+        return
 
-    import numpy as np
-    import torch
-    from pnp_sci import _lib
-    from pnp_sci.device import hwt_to_thw, thw_to_hwt, to_device
     from pnp_sci.synthetic import make_problem
-
     H, W, T, K, tvw, mask_kind, noise = CONFIGS[args.config]
-    print("[bench] generating %s %dx%dx%d" % (args.config, H, W, T), file=sys.stderr, flush=True)
+    log("generating %s %dx%dx%d" % (args.config, H, W, T))
     prob = make_problem(H, W, T, seed=0, mask=mask_kind, noise_rel=noise)
+    head, solve = headline(args, prob)
 
-    L = _lib.lib()
-    y = to_device(prob.y)
-    ps = to_device(prob.phi_sum)
-    phi = to_device(prob.mask_thw)
-    p = _lib.Params()
-    L.scigap_default_params(p, H, W, T)
-    p.iters = K
-    p.tv_weight = tvw
-    p.tv_iter_max = 5
-    nbytes = L.scigap_solve_workspace_bytes(p)
-    ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
-    x = torch.empty((T, H, W), dtype=torch.float32, device="cuda")
-    stats = torch.empty(K, dtype=torch.int32, device="cuda")
-    stream = _lib.stream_ptr()
-
-    def solve():
-        _lib.check(L.scigap_solve(p, y.data_ptr(), phi.data_ptr(), ps.data_ptr(), None,
-                                  x.data_ptr(), None, None, stats.data_ptr(), ws.data_ptr(),
-                                  nbytes, stream))
-
-    for _ in range(args.warmup):
-        solve()
-    torch.cuda.synchronize()
-    print("[bench] timing %d steps" % args.steps, file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        solve()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    value = H * W * T * K / dt
-
-    # Dominant kernel, timed with HIP events on the launch stream.
-    prof = _lib.kernel_profile(solve)
-    n_it, ms_it = prof.get("fused_iteration", (0, 0.0))
-    if n_it == 0:
-        raise RuntimeError("no fused_iteration launches were timed: %s" % prof)
-    bytes_px = (3 * T + 4) * 4                 # accelerated GAP, per pixel per outer iteration
-    avg_it = ms_it / n_it                      # one launch covers all H*W pixels
-    achieved = bytes_px * H * W / (avg_it * 1e-3) / 1e9
-    iter_ms = sum(v[1] for k, v in prof.items() if k.startswith("fused")) / n_it
-    iter_gbs = bytes_px * H * W / (iter_ms * 1e-3) / 1e9
-
-    xo = x.cpu().numpy()
-    mse = float(np.mean((xo.astype(np.float64) - prob.truth_thw) ** 2))
-    quality = {"psnr_db": 10 * math.log10(1.0 / mse),
-               "early_exit_frames": int(stats.sum().item())}
-    if not args.no_check:
-        # FP64 C oracle on the same full-size problem for the first check-iters
-        # outer iterations (the full 60 take minutes of host time).
+    if not args.no_traffic and args.config == "cfg2":
         try:
-            sys.path.insert(0, os.path.join(ROOT, "oracle"))
-            import oracle_c
-            kc = min(args.check_iters, K)
-            print("[bench] FP64 C oracle, %d outer iterations" % kc, file=sys.stderr, flush=True)
-            pc = _lib.Params.from_buffer_copy(p)
-            pc.iters = kc
-            xc = torch.empty_like(x)
-            sc = torch.empty(kc, dtype=torch.int32, device="cuda")
-            _lib.check(L.scigap_solve(pc, y.data_ptr(), phi.data_ptr(), ps.data_ptr(), None,
-                                      xc.data_ptr(), None, None, sc.data_ptr(), ws.data_ptr(),
-                                      nbytes, stream))
-            t1 = time.perf_counter()
-            want = oracle_c.gap_tv(prob.y, prob.mask, prob.phi_sum, iters=kc, tv_weight=tvw,
-                                   tv_iter_max=5)
-            got = thw_to_hwt(xc).cpu().numpy().astype(np.float64)
-            quality["rel_err_vs_fp64_oracle"] = float(
-                np.linalg.norm(got - want) / np.linalg.norm(want))
-            quality["oracle_check"] = ("full %dx%dx%d problem, first %d outer iterations, "
-                                       "FP64 C oracle (%.0f s on the host)"
-                                       % (H, W, T, kc, time.perf_counter() - t1))
-        except ImportError as e:
-            quality["rel_err_vs_fp64_oracle"] = None
-            quality["oracle_check"] = str(e)
+            t = measure_traffic()
+            head["roofline"]["traffic"] = t["bytes"]
+            head["roofline"]["traffic_detail"] = t
+        except Exception as e:  # the profiler is measurement support, not the product
+            head["roofline"]["traffic_error"] = repr(e)[:300]
 
-    cpu = None
-    if not args.no_cpu:
-        print("[bench] CPU baseline (%d outer iterations)" % args.cpu_iters, file=sys.stderr,
-              flush=True)
-        v, secs = cpu_baseline(prob, args.cpu_iters, tvw)
-        cpu = {"value": v, "unit": UNIT, "cores": 1, "kind": "port",
-               "sample": "first %d of %d outer iterations of the same %dx%dx%d solve, NumPy "
-                         "fp32 restatement of the reference path, one thread (%.1f s); "
-                         "host has %d CPUs" % (args.cpu_iters, K, H, W, T, secs,
-                                               os.cpu_count() or 0)}
+    extra = {}
+    if not args.no_batch:
+        extra["batched_queue"] = batched(args)
+    if not args.no_unrolled:
+        extra["unrolled"] = unrolled(args)
+
+    cpu, checks = cpu_leg(args, prob, solve, want_unrolled=not args.no_unrolled and
+                          not args.no_check)
+    head["quality"].update(checks)
 
     out = {
-        "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "metric": METRIC, "value": head["value"], "unit": UNIT, "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
         "config": {"workload": "GAP-TV accelerated, %dx%dx%d, %d outer iterations, "
                                "tv_weight %g, 5 TV iterations, %s mask%s"
                                % (H, W, T, K, tvw, mask_kind,
                                   ", noise 0.01*max(y)" if noise else ""),
                    "H": H, "W": W, "T": T, "outer_iters": K, "tv_iter_max": 5},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_fused (one outer iteration, all tiles)",
-                     "avg_launch_ms": avg_it,
-                     "algorithmic_bytes_per_launch": bytes_px * H * W},
-        "outer_iteration": {"ms": iter_ms, "algorithmic_GBps": iter_gbs,
-                            "frac": iter_gbs / HBM_PEAK_GBS,
-                            "kernels": {k: {"launches": v[0], "avg_ms": v[1] / max(v[0], 1)}
-                                        for k, v in prof.items()}},
-        "quality": quality,
+        "roofline": head["roofline"],
+        "outer_iteration": head["outer_iteration"],
+        "quality": head["quality"],
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
 
 

@@ -136,6 +136,11 @@ typedef struct scigap_params {
   int tile_rows;       /* fused-kernel workgroup height in waves for
                           tv_iter_max = 5: 8, 12 or 16 (0 = 12); a tuning
                           knob with no effect on results                 */
+  int engine;          /* fused-kernel engine for tv_iter_max = 5 (0 =
+                          default): 1 two LDS exchanges per TV update,
+                          2 one exchange per update, 3/4 as 1/2 with
+                          interior and border tiles in separate launches;
+                          a tuning knob with no effect on results        */
 } scigap_params;
 
 /* Fill *p with the reference defaults for a GAP solve of an HxWxT problem:

@@ -4,6 +4,8 @@
 // See gap_fused.hip for the algorithm.
 #pragma once
 
+#include <type_traits>
+
 #include "tv_tile.h"
 
 namespace scigap {
@@ -166,10 +168,10 @@ __device__ __forceinline__ void commit_err(float err, double* dst, double* red64
 // frame (x, Phi); lambda*r, the current Phi and the yb accumulator, each used
 // once per frame, live in LDS as [tile row][128] float planes.
 // LDS of one fused workgroup.
-template <int D, int BY, int MR>
+template <int D, int BY, int MR, bool X1 = false>
 struct FusedLds {
   static constexpr int LH = TileGeom<D, BY, MR>::LH;
-  TileLds<BY> eng;
+  std::conditional_t<X1, TileLdsX1<BY>, TileLds<BY>> eng;
   float2 q[LH * 64], ph[LH * 64], yb[LH * 64], img[LH * 64];
   float red[kEnergyFrames * D];  // per-frame energy partials of this tile
   double red64[BY];
@@ -177,14 +179,15 @@ struct FusedLds {
 
 // One tile of one fused outer iteration (the body of k_fused and of the
 // batched persistent kernel).  Returns without work when the tile's class
-// (interior/border) differs from INSIDE.
-template <int D, int BY, int MR, int MODE, bool INSIDE, bool IQA>
+// (interior/border) differs from INSIDE.  X1 selects the one-barrier TV
+// engine (tv_run_x1).
+template <int D, int BY, int MR, int MODE, bool INSIDE, bool IQA, bool X1 = false>
 __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
-                                           FusedLds<D, BY, MR>& sh) {
+                                           FusedLds<D, BY, MR, X1>& sh) {
   using G = TileGeom<D, BY, MR>;
   constexpr int RY = G::RY;
   constexpr int NH = MR;
-  TileLds<BY>& lds = sh.eng;
+  auto& lds = sh.eng;
   float2* const s_q = sh.q;
   float2* const s_ph = sh.ph;
   float2* const s_yb = sh.yb;
@@ -258,7 +261,11 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
 
     float out[RY][2];
     float acc[NH * 8];
-    tv_run_from_zero<D, RY, BY, NH, !INSIDE>(img, out, D, lds, lp, em, a.weight, a.tw, acc, true);
+    if constexpr (X1)
+      tv_run_x1<D, RY, BY, NH, !INSIDE>(img, out, D, lds, lp, em, a.weight, a.tw, acc, true);
+    else
+      tv_run_from_zero<D, RY, BY, NH, !INSIDE>(img, out, D, lds, lp, em, a.weight, a.tw, acc,
+                                               true);
 
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
@@ -341,16 +348,24 @@ __host__ __device__ inline bool tile_inside(int tile, int tiles_x, int H, int W)
 }
 
 // One launch covers interior and border tiles; the two bodies are separate
-// instantiations (the interior one has no masks or bounds checks).
-template <int D, int BY, int MR, int MODE, bool IQA>
+// instantiations (the interior one has no masks or bounds checks).  PART
+// restricts a launch to one class (1: interior, 2: border; 0: both), so that
+// the interior kernel is allocated registers for its own body only.
+template <int D, int BY, int MR, int MODE, bool IQA, bool X1 = false, int PART = 0>
 __global__ void __launch_bounds__(64 * BY) k_fused(FusedArgs a) {
-  __shared__ FusedLds<D, BY, MR> sh;
+  __shared__ FusedLds<D, BY, MR, X1> sh;
   const int tile = tile_of_block(blockIdx.x, a.per_xcd);
   if (tile >= a.ntiles) return;
-  if (tile_inside<D, BY, MR>(tile, a.tiles_x, a.H, a.W))
-    fused_tile<D, BY, MR, MODE, true, IQA>(a, tile, sh);
-  else
-    fused_tile<D, BY, MR, MODE, false, IQA>(a, tile, sh);
+  const bool inside = tile_inside<D, BY, MR>(tile, a.tiles_x, a.H, a.W);
+  if constexpr (PART != 2) {
+    if (inside) {
+      fused_tile<D, BY, MR, MODE, true, IQA, X1>(a, tile, sh);
+      return;
+    }
+  }
+  if constexpr (PART != 1) {
+    if (!inside) fused_tile<D, BY, MR, MODE, false, IQA, X1>(a, tile, sh);
+  }
 }
 
 // Early-exit decision: one workgroup derives every frame's exit iteration

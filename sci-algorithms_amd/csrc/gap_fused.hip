@@ -64,43 +64,80 @@ unsigned set_tiles(FusedArgs& a) {
   return unsigned(a.per_xcd * 8);
 }
 
+template <int D, int BY, int MODE, bool IQA, bool X1, int PART>
+void launch_kernel(const FusedArgs& a, unsigned grid, hipStream_t s) {
+  k_fused<D, BY, 1, MODE, IQA, X1, PART><<<grid, 64 * BY, 0, s>>>(a);
+}
+
+// Engines of the fused kernel (scigap_params.engine):
+//   1  two LDS exchanges per TV update, interior and border tiles in one launch
+//   2  one exchange per update (tv_run_x1), one launch
+//   3  as 1, interior and border tiles in separate launches
+//   4  as 2, separate launches
+// The alternatives exist for tv_iter_max = 5 (D = 4); other depths run 1.
+// engine 0 picks the default engine for the depth.
+constexpr int kDefaultEngine = 1;
+
 template <int D, int BY, int MODE, bool IQA>
-int launch_main(const FusedArgs& a0, hipStream_t s) {
+int launch_main(const FusedArgs& a0, hipStream_t s, int engine) {
   FusedArgs a = a0;
   const unsigned grid = set_tiles<D, BY, 1>(a);
+  if (engine == 0) engine = kDefaultEngine;
   {
     ProfScope ps(MODE == TV_ONLY ? "tv_fused" : "fused_iteration", s);
-    k_fused<D, BY, 1, MODE, IQA><<<grid, 64 * BY, 0, s>>>(a);
+    if constexpr (D == 4) {
+      constexpr bool x1_fits = sizeof(FusedLds<D, BY, 1, true>) <= 160 * 1024;
+      const bool split = engine == 3 || engine == 4;
+      if (x1_fits && (engine == 2 || engine == 4)) {
+        if constexpr (x1_fits) {
+          if (split) {
+            launch_kernel<D, BY, MODE, IQA, true, 1>(a, grid, s);
+            launch_kernel<D, BY, MODE, IQA, true, 2>(a, grid, s);
+          } else {
+            launch_kernel<D, BY, MODE, IQA, true, 0>(a, grid, s);
+          }
+        }
+      } else if (split) {
+        launch_kernel<D, BY, MODE, IQA, false, 1>(a, grid, s);
+        launch_kernel<D, BY, MODE, IQA, false, 2>(a, grid, s);
+      } else {
+        launch_kernel<D, BY, MODE, IQA, false, 0>(a, grid, s);
+      }
+    } else {
+      launch_kernel<D, BY, MODE, IQA, false, 0>(a, grid, s);
+    }
   }
   SCIGAP_LAUNCH_CHECK("fused iteration launch");
   return SCIGAP_OK;
 }
 
 template <int D, int BY>
-int launch_mode(const FusedArgs& a, hipStream_t s) {
+int launch_mode(const FusedArgs& a, hipStream_t s, int engine) {
   const bool iqa = a.truth != nullptr;
   if (a.mode == GAP_ACC)
-    return iqa ? launch_main<D, BY, GAP_ACC, true>(a, s) : launch_main<D, BY, GAP_ACC, false>(a, s);
+    return iqa ? launch_main<D, BY, GAP_ACC, true>(a, s, engine)
+               : launch_main<D, BY, GAP_ACC, false>(a, s, engine);
   if (a.mode == GAP_PLAIN)
-    return iqa ? launch_main<D, BY, GAP_PLAIN, true>(a, s)
-               : launch_main<D, BY, GAP_PLAIN, false>(a, s);
+    return iqa ? launch_main<D, BY, GAP_PLAIN, true>(a, s, engine)
+               : launch_main<D, BY, GAP_PLAIN, false>(a, s, engine);
   if (a.mode == ADMM)
-    return iqa ? launch_main<D, BY, ADMM, true>(a, s) : launch_main<D, BY, ADMM, false>(a, s);
-  return launch_main<D, BY, TV_ONLY, false>(a, s);
+    return iqa ? launch_main<D, BY, ADMM, true>(a, s, engine)
+               : launch_main<D, BY, ADMM, false>(a, s, engine);
+  return launch_main<D, BY, TV_ONLY, false>(a, s, engine);
 }
 
 template <int D>
-int launch_fused(const FusedArgs& a0, hipStream_t s, int tile_rows) {
+int launch_fused(const FusedArgs& a0, hipStream_t s, int tile_rows, int engine) {
   constexpr int BY = FusedShape<D>::BY, MR = FusedShape<D>::MR;
   int rc;
   if constexpr (D == 4) {
     // 12 waves measured fastest at 2048x2048x20 (558 us against 697 us for
     // 16 and 703 us for 8 per outer iteration).
-    rc = tile_rows == 8    ? launch_mode<4, 8>(a0, s)
-         : tile_rows == 16 ? launch_mode<4, 16>(a0, s)
-                           : launch_mode<4, 12>(a0, s);
+    rc = tile_rows == 8    ? launch_mode<4, 8>(a0, s, engine)
+         : tile_rows == 16 ? launch_mode<4, 16>(a0, s, engine)
+                           : launch_mode<4, 12>(a0, s, engine);
   } else {
-    rc = launch_mode<D, BY>(a0, s);
+    rc = launch_mode<D, BY>(a0, s, engine);
   }
   if (rc) return rc;
   FusedArgs a = a0;
@@ -118,14 +155,15 @@ int launch_fused(const FusedArgs& a0, hipStream_t s, int tile_rows) {
   return SCIGAP_OK;
 }
 
-int launch_fused_depth(int D, const FusedArgs& a, hipStream_t s, int tile_rows = 0) {
+int launch_fused_depth(int D, const FusedArgs& a, hipStream_t s, int tile_rows = 0,
+                       int engine = 0) {
   switch (D) {
-    case 1: return launch_fused<1>(a, s, tile_rows);
-    case 2: return launch_fused<2>(a, s, tile_rows);
-    case 3: return launch_fused<3>(a, s, tile_rows);
-    case 4: return launch_fused<4>(a, s, tile_rows);
-    case 5: return launch_fused<5>(a, s, tile_rows);
-    case 6: return launch_fused<6>(a, s, tile_rows);
+    case 1: return launch_fused<1>(a, s, tile_rows, engine);
+    case 2: return launch_fused<2>(a, s, tile_rows, engine);
+    case 3: return launch_fused<3>(a, s, tile_rows, engine);
+    case 4: return launch_fused<4>(a, s, tile_rows, engine);
+    case 5: return launch_fused<5>(a, s, tile_rows, engine);
+    case 6: return launch_fused<6>(a, s, tile_rows, engine);
   }
   return fail(SCIGAP_EUNSUPPORTED, "fused depth %d not instantiated", D);
 }
@@ -183,6 +221,7 @@ int check_params(const scigap_params* p) {
   if (!(p->tv_weight > 0.f)) return fail(SCIGAP_EINVAL, "tv_weight must be > 0");
   if (p->tile_rows != 0 && p->tile_rows != 8 && p->tile_rows != 12 && p->tile_rows != 16)
     return fail(SCIGAP_EINVAL, "tile_rows must be 0, 8, 12 or 16");
+  if (p->engine < 0 || p->engine > 4) return fail(SCIGAP_EINVAL, "engine must be in [0, 4]");
   if (p->H * p->W >= (int64_t(1) << 30))
     return fail(SCIGAP_EINVAL, "H*W must be < 2^30 (buffer-addressed planes)");
   return SCIGAP_OK;
@@ -260,6 +299,7 @@ void scigap_default_params(scigap_params* p, int64_t H, int64_t W, int64_t T) {
   p->tv_weight_decay = 1.f;
   p->gamma_decay = 1.f;
   p->tile_rows = 0;
+  p->engine = 0;
 }
 
 size_t scigap_solve_workspace_bytes(const scigap_params* p) {
@@ -337,7 +377,7 @@ int scigap_solve(const scigap_params* p, const float* y, const float* phi, const
       ea.x_in = ws.vbuf;
       k_gap_epi<<<grid_pixels(N), 256, 0, s>>>(ea);
       SCIGAP_LAUNCH_CHECK("epilogue launch");
-    } else if (int rc = launch_fused_depth(D, a, s, p->tile_rows)) {
+    } else if (int rc = launch_fused_depth(D, a, s, p->tile_rows, p->engine)) {
       return rc;
     }
   }

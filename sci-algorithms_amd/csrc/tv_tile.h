@@ -204,6 +204,143 @@ __device__ __forceinline__ void tv_run_from_zero(const float2* img, float (&out)
   }
 }
 
+// One-barrier variant of the engine.  Each update publishes, per wave, the
+// new p0 of its last row (the wave below needs it for its first divergence
+// row) and the new p0, p1 of its first row, from which the wave above forms
+// the first `out` row of this wave itself (the same arithmetic as this wave's
+// own divergence, so the values are identical).  One barrier per update
+// instead of two; the slots alternate with the update parity so that a slot
+// is never rewritten before every reader has passed the next barrier.
+template <int BY>
+struct TileLdsX1 {
+  float2 last0[2][BY * 64];
+  float2 first0[2][BY * 64];
+  float2 first1[2][BY * 64];
+};
+
+// Dual update from `out` with the next wave's first `out` row in `below`
+// (no LDS traffic); adds weight*sum|g| per strip half to gacc[h*8].
+template <int RY, int NH, bool EDGE>
+__device__ __forceinline__ void tv_grad_regs(const float (&out)[RY][2], float2 below,
+                                             float (&p0)[RY][2], float (&p1)[RY][2],
+                                             const EdgeMasks& em, float weight, float tw,
+                                             float* gacc) {
+  constexpr float tau = 0.25f;
+  float s[NH] = {};
+#pragma unroll
+  for (int j = 0; j < RY; ++j) {
+    float dn0 = j + 1 < RY ? out[j + 1][0] : below.x;
+    float dn1 = j + 1 < RY ? out[j + 1][1] : below.y;
+    float right1 = from_next_lane(out[j][0]);
+    float g00 = dn0 - out[j][0];
+    float g01 = dn1 - out[j][1];
+    float g10 = out[j][1] - out[j][0];
+    float g11 = right1 - out[j][1];
+    if constexpr (EDGE) {
+      const bool v_ok = (em.rowok0 >> j) & 1u, h_ok = (em.rowin >> j) & 1u;
+      g00 = v_ok ? g00 * em.colin[0] : 0.f;
+      g01 = v_ok ? g01 * em.colin[1] : 0.f;
+      g10 = h_ok ? g10 * em.colok1[0] : 0.f;
+      g11 = h_ok ? g11 * em.colok1[1] : 0.f;
+    }
+    float n0 = fast_sqrt(fmaf(g00, g00, g10 * g10));
+    float n1 = fast_sqrt(fmaf(g01, g01, g11 * g11));
+    const int h = (NH == 2 && j >= RY / 2) ? 1 : 0;
+    s[h] += n0 + n1;
+    float i0 = fast_rcp(fmaf(tw, n0, 1.f));
+    float i1 = fast_rcp(fmaf(tw, n1, 1.f));
+    p0[j][0] = fmaf(-tau, g00, p0[j][0]) * i0;
+    p1[j][0] = fmaf(-tau, g10, p1[j][0]) * i0;
+    p0[j][1] = fmaf(-tau, g01, p0[j][1]) * i1;
+    p1[j][1] = fmaf(-tau, g11, p1[j][1]) * i1;
+    asm("" : "+v"(p0[j][0]), "+v"(p1[j][0]), "+v"(p0[j][1]), "+v"(p1[j][1]));
+  }
+  if (gacc)
+#pragma unroll
+    for (int h = 0; h < NH; ++h) gacc[h * 8] = fmaf(weight, s[h], gacc[h * 8]);
+}
+
+// out = img + d(p) with the p0 row above the strip in `above`.
+template <int RY, int NH>
+__device__ __forceinline__ void tv_div_regs(const float2* img, float (&out)[RY][2],
+                                            const float (&p0)[RY][2], const float (&p1)[RY][2],
+                                            float2 above, float* dacc) {
+  float s[NH] = {};
+#pragma unroll
+  for (int j = 0; j < RY; ++j) {
+    float up0 = j > 0 ? p0[j - 1][0] : above.x;
+    float up1 = j > 0 ? p0[j - 1][1] : above.y;
+    float left0 = from_prev_lane(p1[j][1]);
+    float d0 = (up0 - p0[j][0]) + (left0 - p1[j][0]);
+    float d1 = (up1 - p0[j][1]) + (p1[j][0] - p1[j][1]);
+    const float2 im = img[j * 64];
+    out[j][0] = im.x + d0;
+    out[j][1] = im.y + d1;
+    const int h = (NH == 2 && j >= RY / 2) ? 1 : 0;
+    s[h] = fmaf(d0, d0, s[h]);
+    s[h] = fmaf(d1, d1, s[h]);
+  }
+  if (dacc)
+#pragma unroll
+    for (int h = 0; h < NH; ++h) dacc[h * 8] += s[h];
+}
+
+// tv_run_from_zero with one barrier per update (plus one at entry, after
+// which the img rows of the wave below are read).  Same arguments and
+// results, bit for bit.
+template <int D, int RY, int BY, int NH, bool EDGE>
+__device__ __forceinline__ void tv_run_x1(const float2* img, float (&out)[RY][2], int nup,
+                                          TileLdsX1<BY>& lds, const LanePos& lp,
+                                          const EdgeMasks& em, float weight, float tw,
+                                          float (&acc)[NH * 8], bool accumulate) {
+  float p0[RY][2], p1[RY][2];
+#pragma unroll
+  for (int j = 0; j < RY; ++j) {
+    const float2 im = img[j * 64];
+    out[j][0] = im.x;
+    out[j][1] = im.y;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      p0[j][k] = 0.f;
+      p1[j][k] = 0.f;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NH * 8; ++i) acc[i] = 0.f;
+  const bool has_below = lp.w + 1 < BY;
+  const bool has_above = lp.w > 0;
+  __syncthreads();  // img rows of the wave below
+  const float2 bimg = has_below ? img[RY * 64] : make_float2(0.f, 0.f);
+  float2 below = has_below ? bimg : make_float2(out[RY - 1][0], out[RY - 1][1]);
+  const int me = lp.w * 64 + lp.lane;
+#pragma unroll
+  for (int u = 0; u < D; ++u) {
+    if (u < nup) {
+      tv_grad_regs<RY, NH, EDGE>(out, below, p0, p1, em, weight, tw,
+                                 accumulate ? &acc[u] : nullptr);
+      const int par = u & 1;
+      lds.last0[par][me] = make_float2(p0[RY - 1][0], p0[RY - 1][1]);
+      lds.first0[par][me] = make_float2(p0[0][0], p0[0][1]);
+      lds.first1[par][me] = make_float2(p1[0][0], p1[0][1]);
+      __syncthreads();
+      const float2 above = has_above ? lds.last0[par][me - 64] : make_float2(0.f, 0.f);
+      float2 bp0 = make_float2(0.f, 0.f), bp1 = make_float2(0.f, 0.f);
+      if (has_below) {
+        bp0 = lds.first0[par][me + 64];
+        bp1 = lds.first1[par][me + 64];
+      }
+      const bool dn = accumulate && u + 1 < D;
+      tv_div_regs<RY, NH>(img, out, p0, p1, above, dn ? &acc[u + 1] : nullptr);
+      // first out row of the wave below, as that wave forms it in tv_div_regs
+      const float left0 = from_prev_lane(bp1.y);
+      const float d0 = (p0[RY - 1][0] - bp0.x) + (left0 - bp1.x);
+      const float d1 = (p0[RY - 1][1] - bp0.y) + (bp1.x - bp1.y);
+      below = has_below ? make_float2(bimg.x + d0, bimg.y + d1)
+                        : make_float2(out[RY - 1][0], out[RY - 1][1]);
+    }
+  }
+}
+
 // Frame energies: each wave reduces its lanes' partials (DPP) and adds them
 // into an LDS slot of the frame with an LDS float atomic, so frames need no
 // barrier; energy_flush moves a block of frames to the global fp64

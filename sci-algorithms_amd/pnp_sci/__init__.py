@@ -8,6 +8,13 @@ Public names mirror the reference modules:
   GAP_TV_rec, ADMM_TV_rec
 * ``pnp_sci.utils``: A_, At_, psnr, make_operators
 * ``pnp_sci.tv``: denoise_tv_chambolle (scikit-image signature)
+
+Beyond the reference:
+
+* ``pnp_sci.batch``: BatchPlan / gap_denoise_batch, many measurements of
+  uneven sizes in one persistent kernel per outer iteration (hipGraph replay)
+* ``pnp_sci.unrolled``: unrolled_gap_tv / UnrolledGAPTV, the differentiable
+  unrolled GAP-TV with hand-written backward kernels
 """
 
 from .pnp_sci_algo import (ADMM_TV_rec, GAP_TV_rec, admm_denoise, admmdenoise_cacti,

@@ -48,7 +48,7 @@ class Params(ctypes.Structure):
         ("tv_weight", ctypes.c_float), ("tv_eps", ctypes.c_float),
         ("tv_iter_max", ctypes.c_int),
         ("tv_weight_decay", ctypes.c_float), ("gamma_decay", ctypes.c_float),
-        ("tile_rows", ctypes.c_int),
+        ("tile_rows", ctypes.c_int), ("engine", ctypes.c_int),
     ]
 
 
