@@ -110,6 +110,21 @@ int scigap_tv_chambolle(const float* image, float* out,
                         int* stop_iter, void* workspace, size_t workspace_bytes,
                         void* stream);
 
+/* The same TV with an explicit fusion depth: `depth` dual updates per launch
+ * (1..6), continuing from the stored dual field between launches, with the
+ * early exit applied after each launch and frames that stopped inside it
+ * recomputed.  depth 0: all n_iter_max-1 updates in one launch when they are
+ * at most 6, else the autotuned depth (6 when untuned).  Results do not
+ * depend on the depth.  The workspace for a given depth is
+ * scigap_tv_workspace_bytes_depth(); depth 1 needs the most. */
+size_t scigap_tv_workspace_bytes_depth(int64_t H, int64_t W, int64_t T, int n_iter_max,
+                                       int depth);
+int scigap_tv_chambolle_depth(const float* image, float* out,
+                              int64_t H, int64_t W, int64_t T,
+                              float weight, float eps, int n_iter_max, int depth,
+                              int* stop_iter, void* workspace, size_t workspace_bytes,
+                              void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* GAP-TV / ADMM-TV solver                                                */
 /* ---------------------------------------------------------------------- */
@@ -271,6 +286,40 @@ int scigap_unrolled_backward(const scigap_unrolled* u, const float* lambdas,
                              float* grad_mask, float* grad_meas, double* grad_lambda,
                              double* grad_weight, void* workspace, size_t workspace_bytes,
                              void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Autotuning (no reference counterpart)                                  */
+/* ---------------------------------------------------------------------- */
+
+typedef struct scigap_tune_result {
+  int tile_rows;      /* chosen fused-kernel tile height (solve)            */
+  int engine;         /* chosen fused-kernel engine (solve)                 */
+  int tv_depth;       /* chosen TV fusion depth (TV)                        */
+  int candidates;     /* configurations timed                               */
+  double best_ms;     /* per outer iteration (solve) or per call (TV)       */
+  double default_ms;  /* the same for the first candidate (the default)     */
+} scigap_tune_result;
+
+/* Time the fused kernel's tile heights and engines on this problem (`iters`
+ * outer iterations each; x_scratch is a (T,H,W) buffer the tuner
+ * overwrites), remember the fastest for the shape, and return it.
+ * Synchronises `stream`.  Later scigap_solve calls with tile_rows = 0 and
+ * engine = 0 use the remembered choice.  The choice never changes results. */
+int scigap_autotune_solve(const scigap_params* p, const float* y, const float* phi,
+                          const float* phi_sum, float* x_scratch, int iters,
+                          void* workspace, size_t workspace_bytes, void* stream,
+                          scigap_tune_result* result);
+
+/* Time every TV fusion depth for this shape (workspace of at least
+ * scigap_tv_workspace_bytes_depth(H,W,T,n_iter_max,1) bytes), remember the
+ * fastest (used by depth 0 for deep TV and by the solvers' deep-TV path).
+ * Synchronises `stream`. */
+int scigap_autotune_tv(const float* image, float* out, int64_t H, int64_t W, int64_t T,
+                       float weight, float eps, int n_iter_max, void* workspace,
+                       size_t workspace_bytes, void* stream, scigap_tune_result* result);
+
+/* Forget every tuned choice (in this process; the cache file is kept). */
+int scigap_tune_clear(void);
 
 /* ---------------------------------------------------------------------- */
 /* Kernel timing (measurement support; no reference counterpart)          */

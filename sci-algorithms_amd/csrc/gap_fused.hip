@@ -35,9 +35,11 @@
 namespace scigap {
 
 int fill_f64(double* p, int n, double v, hipStream_t s);
-size_t tv_general_workspace(int64_t H, int64_t W, int64_t T, int n);
-int tv_general(const float* image, float* out, int64_t H, int64_t W, int64_t T, float weight,
-               float eps, int n, int* stop_iter, void* ws, hipStream_t s);
+size_t tv_chunked_workspace(int64_t H, int64_t W, int64_t T, int n, int nchunks_max);
+int tv_chunked(const float* image, float* out, int64_t H, int64_t W, int64_t T, float weight,
+               float eps, int n, int depth, int* stop_iter, void* ws, hipStream_t s);
+int tuned_tv_depth(int64_t H, int64_t W, int64_t T, int n);
+bool tuned_solve_config(const scigap_params& p, int* tile_rows, int* engine);
 
 namespace {
 
@@ -203,7 +205,7 @@ Workspace carve(const scigap_params* p, char* base) {
   w.st.nexit = reinterpret_cast<int*>(take(K * 4));
   if (p->tv_iter_max - 1 > kMaxFused) {
     w.vbuf = reinterpret_cast<float*>(take(T * N * 4));
-    w.tvws = take(tv_general_workspace(p->H, p->W, p->T, p->tv_iter_max));
+    w.tvws = take(tv_chunked_workspace(p->H, p->W, p->T, p->tv_iter_max, p->tv_iter_max));
   }
   w.bytes = off;
   return w;
@@ -331,6 +333,9 @@ int scigap_solve(const scigap_params* p, const float* y, const float* phi, const
 
   const int D = p->tv_iter_max - 1;
   const bool deep = D > kMaxFused;
+  const int deep_depth = deep ? tuned_tv_depth(p->H, p->W, p->T, p->tv_iter_max) : 0;
+  int tile_rows = p->tile_rows, engine = p->engine;
+  if (tile_rows == 0 && engine == 0) tuned_solve_config(*p, &tile_rows, &engine);
   hipStream_t s = as_stream(stream);
   const int K = p->iters;
   const int mode = solver_mode(*p);
@@ -370,14 +375,14 @@ int scigap_solve(const scigap_params* p, const float* y, const float* phi, const
       pa.x_out = ws.vbuf;
       k_gap_proj<<<grid_pixels(N), 256, 0, s>>>(pa);
       SCIGAP_LAUNCH_CHECK("projection launch");
-      if (int rc = tv_general(ws.vbuf, a.x_out, p->H, p->W, T, a.weight, p->tv_eps,
-                              p->tv_iter_max, nullptr, ws.tvws, s))
+      if (int rc = tv_chunked(ws.vbuf, a.x_out, p->H, p->W, T, a.weight, p->tv_eps,
+                              p->tv_iter_max, deep_depth, nullptr, ws.tvws, s))
         return rc;
       FusedArgs ea = a;
       ea.x_in = ws.vbuf;
       k_gap_epi<<<grid_pixels(N), 256, 0, s>>>(ea);
       SCIGAP_LAUNCH_CHECK("epilogue launch");
-    } else if (int rc = launch_fused_depth(D, a, s, p->tile_rows, p->engine)) {
+    } else if (int rc = launch_fused_depth(D, a, s, tile_rows, engine)) {
       return rc;
     }
   }
@@ -394,25 +399,35 @@ int scigap_solve(const scigap_params* p, const float* y, const float* phi, const
   return SCIGAP_OK;
 }
 
-size_t scigap_tv_workspace_bytes(int64_t H, int64_t W, int64_t T, int n_iter_max) {
-  if (check_shape(H, W, T) != SCIGAP_OK || n_iter_max < 1) return 0;
+size_t scigap_tv_workspace_bytes_depth(int64_t H, int64_t W, int64_t T, int n_iter_max,
+                                       int depth) {
+  if (check_shape(H, W, T) != SCIGAP_OK || n_iter_max < 1 || depth < 0 || depth > kMaxFused)
+    return 0;
   const int D = n_iter_max - 1;
   if (D == 0) return 256;
-  if (D <= kMaxFused) return align256(size_t(T) * D * 8) + align256(size_t(T) * 4) + 256;
-  return tv_general_workspace(H, W, T, n_iter_max);
+  if (depth == 0 && D <= kMaxFused) depth = D;
+  if (depth >= D) return align256(size_t(T) * D * 8) + align256(size_t(T) * 4) + 256;
+  return tv_chunked_workspace(H, W, T, n_iter_max, D);
 }
 
-int scigap_tv_chambolle(const float* image, float* out, int64_t H, int64_t W, int64_t T,
-                        float weight, float eps, int n_iter_max, int* stop_iter, void* workspace,
-                        size_t workspace_bytes, void* stream) {
+size_t scigap_tv_workspace_bytes(int64_t H, int64_t W, int64_t T, int n_iter_max) {
+  return scigap_tv_workspace_bytes_depth(H, W, T, n_iter_max, 0);
+}
+
+int scigap_tv_chambolle_depth(const float* image, float* out, int64_t H, int64_t W, int64_t T,
+                              float weight, float eps, int n_iter_max, int depth, int* stop_iter,
+                              void* workspace, size_t workspace_bytes, void* stream) {
   clear_error();
   if (int rc = check_shape(H, W, T)) return rc;
   if (!image || !out) return fail(SCIGAP_ENULL, "scigap_tv_chambolle: null image/out");
   if (n_iter_max < 1) return fail(SCIGAP_EINVAL, "n_iter_max must be >= 1");
   if (!(weight > 0.f)) return fail(SCIGAP_EINVAL, "weight must be > 0");
+  if (depth < 0 || depth > kMaxFused) return fail(SCIGAP_EINVAL, "depth must be in [0, 6]");
+  if (H * W >= (int64_t(1) << 30))
+    return fail(SCIGAP_EINVAL, "H*W must be < 2^30 (buffer-addressed planes)");
   const size_t vol = size_t(H * W * T) * 4;
   if (overlaps(image, vol, out, vol)) return fail(SCIGAP_EALIAS, "image and out overlap");
-  const size_t need = scigap_tv_workspace_bytes(H, W, T, n_iter_max);
+  const size_t need = scigap_tv_workspace_bytes_depth(H, W, T, n_iter_max, depth);
   if (!workspace || workspace_bytes < need)
     return fail(SCIGAP_EWORKSPACE, "workspace needs %zu bytes, got %zu", need, workspace_bytes);
   if ((reinterpret_cast<uintptr_t>(image) | reinterpret_cast<uintptr_t>(out)) % 8 != 0)
@@ -429,8 +444,11 @@ int scigap_tv_chambolle(const float* image, float* out, int64_t H, int64_t W, in
     }
     return SCIGAP_OK;
   }
-  if (D > kMaxFused)
-    return tv_general(image, out, H, W, T, weight, eps, n_iter_max, stop_iter, workspace, s);
+  if (depth == 0) depth = D <= kMaxFused ? D : tuned_tv_depth(H, W, T, n_iter_max);
+  if (depth < D)
+    return tv_chunked(image, out, H, W, T, weight, eps, n_iter_max, depth, stop_iter, workspace,
+                      s);
+  // All updates in one launch from p = 0.
   double* energy = static_cast<double*>(workspace);
   char* rest = static_cast<char*>(workspace) + align256(size_t(T) * D * 8);
   if (int rc = fill_f64(energy, int(T) * D, 0.0, s)) return rc;
@@ -448,6 +466,13 @@ int scigap_tv_chambolle(const float* image, float* out, int64_t H, int64_t W, in
   a.eps = eps;
   a.mode = TV_ONLY;
   return launch_fused_depth(D, a, s);
+}
+
+int scigap_tv_chambolle(const float* image, float* out, int64_t H, int64_t W, int64_t T,
+                        float weight, float eps, int n_iter_max, int* stop_iter, void* workspace,
+                        size_t workspace_bytes, void* stream) {
+  return scigap_tv_chambolle_depth(image, out, H, W, T, weight, eps, n_iter_max, 0, stop_iter,
+                                   workspace, workspace_bytes, stream);
 }
 
 }  // extern "C"

@@ -14,7 +14,7 @@ import threading
 from .build import LIB_PATH
 
 __all__ = ["ScigapError", "ScigapUnavailable", "Params", "KernelStat", "BatchItem",
-           "BatchOptions", "Unrolled", "lib", "check",
+           "BatchOptions", "Unrolled", "TuneResult", "lib", "check",
            "stream_ptr", "kernel_profile", "SYMBOLS", "METHOD_GAP", "METHOD_ADMM"]
 
 METHOD_GAP = 0
@@ -75,6 +75,14 @@ class Unrolled(ctypes.Structure):
                 ("logits", ctypes.c_int), ("measure", ctypes.c_int)]
 
 
+class TuneResult(ctypes.Structure):
+    """Mirror of ``scigap_tune_result``."""
+
+    _fields_ = [("tile_rows", ctypes.c_int), ("engine", ctypes.c_int), ("tv_depth", ctypes.c_int),
+                ("candidates", ctypes.c_int), ("best_ms", ctypes.c_double),
+                ("default_ms", ctypes.c_double)]
+
+
 class KernelStat(ctypes.Structure):
     """Mirror of ``scigap_kernel_stat``."""
 
@@ -100,6 +108,10 @@ SYMBOLS = {
     "scigap_tv_workspace_bytes": (ctypes.c_size_t, [_I64, _I64, _I64, _INT]),
     "scigap_tv_chambolle": (_INT, [_P, _P, _I64, _I64, _I64, ctypes.c_float, ctypes.c_float,
                                    _INT, _P, _P, ctypes.c_size_t, _P]),
+    "scigap_tv_workspace_bytes_depth": (ctypes.c_size_t, [_I64, _I64, _I64, _INT, _INT]),
+    "scigap_tv_chambolle_depth": (_INT, [_P, _P, _I64, _I64, _I64, ctypes.c_float,
+                                         ctypes.c_float, _INT, _INT, _P, _P, ctypes.c_size_t,
+                                         _P]),
     "scigap_default_params": (None, [ctypes.POINTER(Params), _I64, _I64, _I64]),
     "scigap_solve_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(Params)]),
     "scigap_solve": (_INT, [ctypes.POINTER(Params), _P, _P, _P, _P, _P, _P, _P, _P, _P,
@@ -119,6 +131,11 @@ SYMBOLS = {
     "scigap_unrolled_backward": (_INT, [ctypes.POINTER(Unrolled), _P, _P, _P, _P, _P,
                                         ctypes.c_size_t, _P, _P, _P, _P, _P, _P,
                                         ctypes.c_size_t, _P]),
+    "scigap_autotune_solve": (_INT, [ctypes.POINTER(Params), _P, _P, _P, _P, _INT, _P,
+                                     ctypes.c_size_t, _P, ctypes.POINTER(TuneResult)]),
+    "scigap_autotune_tv": (_INT, [_P, _P, _I64, _I64, _I64, ctypes.c_float, ctypes.c_float, _INT,
+                                  _P, ctypes.c_size_t, _P, ctypes.POINTER(TuneResult)]),
+    "scigap_tune_clear": (_INT, []),
     "scigap_profile_begin": (_INT, []),
     "scigap_profile_end": (_INT, [ctypes.POINTER(KernelStat), _INT, ctypes.POINTER(_INT)]),
 }

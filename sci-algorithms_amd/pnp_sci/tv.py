@@ -16,22 +16,25 @@ from .device import hwt_to_thw, thw_to_hwt, to_device, to_host_like
 __all__ = ["denoise_tv_chambolle", "tv_device"]
 
 
-def tv_device(image_thw, weight=0.1, eps=2e-4, n_iter_max=200, return_stop=False):
+def tv_device(image_thw, weight=0.1, eps=2e-4, n_iter_max=200, return_stop=False, depth=0):
     """TV of a (T,H,W) float32 device tensor; returns a new tensor (and the
-    per-frame iteration whose image was returned, if asked)."""
+    per-frame iteration whose image was returned, if asked).  ``depth`` is
+    the number of dual updates fused per launch (1..6; 0 = automatic), a
+    tuning knob that does not change the result."""
     import torch
     T, H, W = image_thw.shape
     L = _lib.lib()
-    nbytes = L.scigap_tv_workspace_bytes(H, W, T, int(n_iter_max))
+    nbytes = L.scigap_tv_workspace_bytes_depth(H, W, T, int(n_iter_max), int(depth))
     if nbytes == 0:
-        raise ValueError("invalid TV arguments (n_iter_max must be >= 1)")
+        raise ValueError("invalid TV arguments (n_iter_max must be >= 1, depth in [0, 6])")
     ws = torch.empty(nbytes, dtype=torch.uint8, device=image_thw.device)
     out = torch.empty_like(image_thw)
     stop = torch.empty(T, dtype=torch.int32, device=image_thw.device) if return_stop else None
-    _lib.check(L.scigap_tv_chambolle(image_thw.data_ptr(), out.data_ptr(), H, W, T,
-                                     float(weight), float(eps), int(n_iter_max),
-                                     stop.data_ptr() if stop is not None else None,
-                                     ws.data_ptr(), nbytes, _lib.stream_ptr()))
+    _lib.check(L.scigap_tv_chambolle_depth(image_thw.data_ptr(), out.data_ptr(), H, W, T,
+                                           float(weight), float(eps), int(n_iter_max),
+                                           int(depth),
+                                           stop.data_ptr() if stop is not None else None,
+                                           ws.data_ptr(), nbytes, _lib.stream_ptr()))
     return (out, stop) if return_stop else out
 
 

@@ -356,3 +356,51 @@ def test_engines_and_tile_heights_agree(gpu, method):
     want = solver(pr.y.astype(np.float64), pr.phi_sum.astype(np.float64), lambda v: O.A_(v, m),
                   lambda v: O.At_(v, m), iter_max=6, show_iqa=False)[0]
     assert rel(np.moveaxis(ref, 0, -1), want) <= 1e-4
+
+
+@pytest.mark.parametrize("n", [5, 12, 30])
+def test_tv_fusion_depths_agree_with_oracle(gpu, n):
+    """Every TV fusion depth (1..6 dual updates per launch, the dual field
+    carried between launches) returns the oracle's frames and exit
+    iterations, including frames that stop inside a launch."""
+    from pnp_sci.device import hwt_to_thw, thw_to_hwt, to_device
+    from pnp_sci.tv import tv_device
+    r = np.arange(96)[:, None]
+    c = np.arange(200)[None, :]
+    smooth = 0.5 + 0.2 * np.cos(2 * np.pi * (r / 96.0 + c / 400.0))
+    noisy = np.random.default_rng(n).random((96, 200))
+    img = np.stack([smooth, noisy, noisy[::-1]], -1).astype(np.float32)
+    for eps in (2e-4, 3e-2):
+        want, stops_want = [], []
+        for t in range(3):
+            st = {}
+            want.append(O.tv_chambolle_nd(img[..., t].astype(np.float64), 0.1, eps, n, _stats=st))
+            stops_want.append(st["returned"])
+        for depth in range(1, 7):
+            out, stop = tv_device(hwt_to_thw(to_device(img)), 0.1, eps, n, return_stop=True,
+                                  depth=depth)
+            got = thw_to_hwt(out).cpu().numpy()
+            assert stop.cpu().tolist() == stops_want, (eps, depth)
+            for t in range(3):
+                assert np.max(np.abs(got[..., t] - want[t])) <= 2e-5, (eps, depth, t)
+
+
+def test_autotuner_choices_and_cache(gpu, tmp_path):
+    import torch
+    from pnp_sci import tune
+    from pnp_sci.device import to_device
+    from pnp_sci.synthetic import make_problem
+    from pnp_sci.pnp_sci_algo import solve_device
+    tune.clear()
+    pr = make_problem(200, 328, 6, seed=4, mask="shift")
+    y, phi, ps = to_device(pr.y), to_device(pr.mask_thw), to_device(pr.phi_sum)
+    before, _, _ = solve_device(y, phi, ps, iters=5)
+    r = tune.autotune_solve(y, phi, ps, iters=2)
+    assert r["candidates"] == 12 and r["tile_rows"] in (8, 12, 16) and 1 <= r["engine"] <= 4
+    assert r["best_ms"] <= r["default_ms"]
+    after, _, _ = solve_device(y, phi, ps, iters=5)      # uses the tuned choice
+    assert rel(after.cpu().numpy(), before.cpu().numpy()) <= 1e-6
+    img = torch.rand((3, 96, 200), device="cuda")
+    t = tune.autotune_tv(img, n_iter_max=14)
+    assert t["candidates"] == 6 and 1 <= t["tv_depth"] <= 6
+    tune.clear()
