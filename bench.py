@@ -391,7 +391,9 @@ def cpu_leg(args, prob, solve, want_unrolled):
                "kind": "port",
                "sample": "first %d of %d outer iterations of the same %dx%dx%d solve, NumPy "
                          "fp32 restatement of the reference path, one thread (%.1f s); host has "
-                         "%d CPUs" % (args.cpu_iters, K, H, W, T, secs, os.cpu_count() or 0)}
+                         "%d CPUs, %d in this process's affinity set"
+                         % (args.cpu_iters, K, H, W, T, secs, os.cpu_count() or 0,
+                            len(os.sched_getaffinity(0)))}
     return cpu, checks
 
 
@@ -401,7 +403,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
-    ap.add_argument("--cpu-iters", type=int, default=2,
+    ap.add_argument("--cpu-iters", type=int, default=4,
                     help="outer iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 oracle comparisons")

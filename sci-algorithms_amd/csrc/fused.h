@@ -55,7 +55,7 @@ __device__ __forceinline__ int tile_of_block(int bid, int per_xcd) {
 // base + 32-bit lane offset + uniform (scalar) offset and returns 0 (or drops
 // the store) outside [0, N*4) bytes, so a plane needs one VGPR of address per
 // lane and an out-of-range offset can never fault.  Planes are < 4 GiB
-// (H*W < 2^30, checked by the host).
+// (H*W <= 2^30 - 8, checked by the host).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float* p, int N) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, unsigned(N) * 4u,
                                            0x00020000);
@@ -164,18 +164,32 @@ __device__ __forceinline__ void commit_err(float err, double* dst, double* red64
 
 // Main pass.  INSIDE selects the interior tiles (no masks, no bounds checks)
 // or the border ring; IQA adds the squared error against the truth.
-// Registers hold the stencil state (img, out, p0, p1) and the prefetched next
-// frame (x, Phi); lambda*r, the current Phi and the yb accumulator, each used
-// once per frame, live in LDS as [tile row][128] float planes.
-// LDS of one fused workgroup.
+// Registers hold the stencil state (out, p0, p1), lambda*r, the current Phi,
+// the yb accumulator and the prefetched next frame (x, Phi); the TV input
+// img lives in LDS as a [tile row][128] float plane (the engine reads it on
+// every divergence).
 template <int D, int BY, int MR, bool X1 = false>
 struct FusedLds {
   static constexpr int LH = TileGeom<D, BY, MR>::LH;
   std::conditional_t<X1, TileLdsX1<BY>, TileLds<BY>> eng;
-  float2 q[LH * 64], ph[LH * 64], yb[LH * 64], img[LH * 64];
+  float2 img[LH * 64];
   float red[kEnergyFrames * D];  // per-frame energy partials of this tile
   double red64[BY];
 };
+
+// Byte offset of pixel (j,k) of the lane's block for a buffer store, or an
+// offset past the end of any plane (the store is dropped) when the lane does
+// not own the pixel.  Planes hold fewer than 2^30 - 8 floats (checked by the
+// host), so 0xFFFFFFF0 is always out of range.
+template <int D, int BY, int MR>
+__device__ __forceinline__ unsigned owned_offset(const TilePlace<D, BY, MR>& tp, int j, int k,
+                                                 int W) {
+  return tp.owns(j, k) ? unsigned(tp.off0 + j * W + k) * 4u : 0xFFFFFFF0u;
+}
+__device__ __forceinline__ void st_at(float* p, int N, unsigned byte_off, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), plane_rsrc(p, N), byte_off, 0,
+                                        0);
+}
 
 // One tile of one fused outer iteration (the body of k_fused and of the
 // batched persistent kernel).  Returns without work when the tile's class
@@ -188,10 +202,6 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
   constexpr int RY = G::RY;
   constexpr int NH = MR;
   auto& lds = sh.eng;
-  float2* const s_q = sh.q;
-  float2* const s_ph = sh.ph;
-  float2* const s_yb = sh.yb;
-  float2* const s_img = sh.img;
   float* const red = sh.red;
   double* const red64 = sh.red64;
   const int H = a.H, W = a.W, T = a.T;
@@ -207,22 +217,25 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
   if constexpr (!INSIDE) em.init<RY>(lp, H, W);
 
   const bool from_y = a.from_y;
+  // q = lambda*r (or y + lambda*r when X_0 = At(y)) over the load tile.
+  float q[RY][2], yb[RY][2];
   if constexpr (MODE != TV_ONLY) {
-    // q = lambda*r (or y + lambda*r when X_0 = At(y)) over the load tile.
-    float q[RY][2];
     load_plane<RY, INSIDE>(a.r_in, q, lp, off0, H, W);
     if (from_y) {
       float yv[RY][2];
       load_plane<RY, INSIDE>(a.y, yv, lp, off0, H, W);
 #pragma unroll
       for (int j = 0; j < RY; ++j)
-        s_q[sl0 + j * 64] = make_float2(fmaf(a.lam, q[j][0], yv[j][0]), fmaf(a.lam, q[j][1], yv[j][1]));
+#pragma unroll
+        for (int k = 0; k < 2; ++k) q[j][k] = fmaf(a.lam, q[j][k], yv[j][k]);
     } else {
 #pragma unroll
-      for (int j = 0; j < RY; ++j) s_q[sl0 + j * 64] = make_float2(a.lam * q[j][0], a.lam * q[j][1]);
+      for (int j = 0; j < RY; ++j)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) q[j][k] *= a.lam;
     }
 #pragma unroll
-    for (int j = 0; j < RY; ++j) s_yb[sl0 + j * 64] = make_float2(0.f, 0.f);
+    for (int j = 0; j < RY; ++j) yb[j][0] = yb[j][1] = 0.f;
   }
   for (int i = threadIdx.x; i < kEnergyFrames * D; i += blockDim.x) red[i] = 0.f;
   float err = 0.f;
@@ -239,19 +252,21 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
   const float* trt = a.truth;
   float* bt = a.b;
   double* en = a.energy;
-  // Every lane reads back only the LDS slots it wrote itself, so the planes
-  // s_q, s_ph, s_img and s_yb need no barriers.
-  float2* const img = s_img + sl0;
+  // Every lane reads back only the LDS slots it wrote itself (the one-barrier
+  // engine synchronises before it reads a neighbour's rows).
+  float2* const img = sh.img + sl0;
+  const bool owns_any = tp.own_r != 0;  // wave-uniform: halo waves own nothing
   for (int t = 0; t < T; ++t) {
+    float ph[RY][2];
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
       if constexpr (MODE == TV_ONLY) {
         img[j * 64] = make_float2(nx[j][0], nx[j][1]);
       } else {
-        const float2 qj = s_q[sl0 + j * 64];
-        s_ph[sl0 + j * 64] = make_float2(nph[j][0], nph[j][1]);
-        img[j * 64] = make_float2(from_y ? qj.x * nph[j][0] : fmaf(qj.x, nph[j][0], nx[j][0]),
-                                  from_y ? qj.y * nph[j][1] : fmaf(qj.y, nph[j][1], nx[j][1]));
+#pragma unroll
+        for (int k = 0; k < 2; ++k) ph[j][k] = nph[j][k];
+        img[j * 64] = make_float2(from_y ? q[j][0] * ph[j][0] : fmaf(q[j][0], ph[j][0], nx[j][0]),
+                                  from_y ? q[j][1] * ph[j][1] : fmaf(q[j][1], ph[j][1], nx[j][1]));
       }
     }
     if (t + 1 < T) {
@@ -267,45 +282,39 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
       tv_run_from_zero<D, RY, BY, NH, !INSIDE>(img, out, D, lds, lp, em, a.weight, a.tw, acc,
                                                true);
 
+    if (owns_any) {
 #pragma unroll
-    for (int j = 0; j < RY; ++j) {
-      if (!((tp.own_r >> j) & 1u)) continue;
-      if constexpr (MODE == TV_ONLY) {
+      for (int j = 0; j < RY; ++j) {
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
-          if (tp.own_c[k]) st1(xo, N, unsigned(off0 + j * W + k), out[j][k]);
-        continue;
-      }
-      const float2 ph = s_ph[sl0 + j * 64];
-      const float2 im = img[j * 64];
-      float2 yb = s_yb[sl0 + j * 64];
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        if (!tp.own_c[k]) continue;
-        const unsigned o = unsigned(off0 + j * W + k);
-        const float f = k ? ph.y : ph.x;
-        const float v = k ? im.y : im.x;
-        float& ybk = k ? yb.y : yb.x;
-        float xn = out[j][k];
-        if constexpr (MODE == ADMM) {
-          if (a.clip) xn = fminf(fmaxf(xn, 0.f), 1.f);
-          if constexpr (IQA) {  // x_k = v_k + b_k
-            float bk = a.read_b ? ld1(bt, N, o) : 0.f;
-            float e = ld1(trt, N, o) - (v + bk);
-            err = fmaf(e, e, err);
+        for (int k = 0; k < 2; ++k) {
+          const unsigned so = owned_offset(tp, j, k, W);
+          float xn = out[j][k];
+          if constexpr (MODE == ADMM) {
+            const float2 im = img[j * 64];
+            const float v = k ? im.y : im.x;
+            if (a.clip) xn = fminf(fmaxf(xn, 0.f), 1.f);
+            if constexpr (IQA) {  // x_k = v_k + b_k
+              if (tp.owns(j, k)) {
+                const unsigned o = unsigned(off0 + j * W + k);
+                float bk = a.read_b ? ld1(bt, N, o) : 0.f;
+                float e = ld1(trt, N, o) - (v + bk);
+                err = fmaf(e, e, err);
+              }
+            }
+            if (a.write_b) st_at(bt, N, so, xn - v);
+            yb[j][k] = fmaf(ph[j][k], 2.f * xn - v, yb[j][k]);
+          } else if constexpr (MODE != TV_ONLY) {
+            if constexpr (IQA) {
+              if (tp.owns(j, k)) {
+                float e = ld1(trt, N, unsigned(off0 + j * W + k)) - xn;
+                err = fmaf(e, e, err);
+              }
+            }
+            yb[j][k] = fmaf(ph[j][k], xn, yb[j][k]);
           }
-          if (a.write_b) st1(bt, N, o, xn - v);
-          ybk = fmaf(f, 2.f * xn - v, ybk);
-        } else {
-          if constexpr (IQA) {
-            float e = ld1(trt, N, o) - xn;
-            err = fmaf(e, e, err);
-          }
-          ybk = fmaf(f, xn, ybk);
+          st_at(xo, N, so, xn);
         }
-        st1(xo, N, o, xn);
       }
-      s_yb[sl0 + j * 64] = yb;
     }
     const int tb = t % kEnergyFrames;
     energy_add<D, BY, MR, NH>(acc, tp.lane_owner, red + tb * D, lp);
@@ -320,16 +329,15 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
     if constexpr (MODE == ADMM) bt += (bt ? N : 0);
   }
 
-  if (MODE != TV_ONLY) {
+  if constexpr (MODE != TV_ONLY) {
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
       if (!((tp.own_r >> j) & 1u)) continue;
-      const float2 yb = s_yb[sl0 + j * 64];
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         if (!tp.own_c[k]) continue;
         const unsigned o = unsigned(off0 + j * W + k);
-        const float v = k ? yb.y : yb.x;
+        const float v = yb[j][k];
         if (a.yb) st1(a.yb, N, o, v);
         if (a.epilogue) epilogue_pixel<MODE>(a, o, v);
       }

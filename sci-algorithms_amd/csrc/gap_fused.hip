@@ -224,8 +224,8 @@ int check_params(const scigap_params* p) {
   if (p->tile_rows != 0 && p->tile_rows != 8 && p->tile_rows != 12 && p->tile_rows != 16)
     return fail(SCIGAP_EINVAL, "tile_rows must be 0, 8, 12 or 16");
   if (p->engine < 0 || p->engine > 4) return fail(SCIGAP_EINVAL, "engine must be in [0, 4]");
-  if (p->H * p->W >= (int64_t(1) << 30))
-    return fail(SCIGAP_EINVAL, "H*W must be < 2^30 (buffer-addressed planes)");
+  if (p->H * p->W > (int64_t(1) << 30) - 8)
+    return fail(SCIGAP_EINVAL, "H*W must be <= 2^30 - 8 (buffer-addressed planes)");
   return SCIGAP_OK;
 }
 
@@ -423,8 +423,8 @@ int scigap_tv_chambolle_depth(const float* image, float* out, int64_t H, int64_t
   if (n_iter_max < 1) return fail(SCIGAP_EINVAL, "n_iter_max must be >= 1");
   if (!(weight > 0.f)) return fail(SCIGAP_EINVAL, "weight must be > 0");
   if (depth < 0 || depth > kMaxFused) return fail(SCIGAP_EINVAL, "depth must be in [0, 6]");
-  if (H * W >= (int64_t(1) << 30))
-    return fail(SCIGAP_EINVAL, "H*W must be < 2^30 (buffer-addressed planes)");
+  if (H * W > (int64_t(1) << 30) - 8)
+    return fail(SCIGAP_EINVAL, "H*W must be <= 2^30 - 8 (buffer-addressed planes)");
   const size_t vol = size_t(H * W * T) * 4;
   if (overlaps(image, vol, out, vol)) return fail(SCIGAP_EALIAS, "image and out overlap");
   const size_t need = scigap_tv_workspace_bytes_depth(H, W, T, n_iter_max, depth);

@@ -437,8 +437,8 @@ UWorkspace carve(const scigap_unrolled& u, char* base) {
 int check_unrolled(const scigap_unrolled* u) {
   if (!u) return fail(SCIGAP_ENULL, "unrolled descriptor is NULL");
   if (int rc = check_shape(u->H, u->W, u->T)) return rc;
-  if (u->H * u->W >= (int64_t(1) << 30))
-    return fail(SCIGAP_EINVAL, "H*W must be < 2^30 (buffer-addressed planes)");
+  if (u->H * u->W > (int64_t(1) << 30) - 8)
+    return fail(SCIGAP_EINVAL, "H*W must be <= 2^30 - 8 (buffer-addressed planes)");
   if (u->batch < 1) return fail(SCIGAP_EINVAL, "batch must be >= 1");
   if (u->iters < 1) return fail(SCIGAP_EINVAL, "iters must be >= 1");
   if (u->tv_iter_max < 2 || u->tv_iter_max > 7)

@@ -19,7 +19,7 @@
 //
 // A chunk moves img + p in (3 planes) and p out (2 planes) per frame pixel,
 // against about 6 planes per update for one launch per update.  The chunk
-// depth is a tuning parameter (scigap_tv_chunk_depth, picked by the
+// depth is a tuning parameter (scigap_tv_chambolle_depth, picked by the
 // autotuner); results do not depend on it.
 
 #include "fused.h"
@@ -49,8 +49,9 @@ struct ChunkArgs {
   int tiles_x, ntiles, per_xcd;
 };
 
+// Waves per workgroup for a tile geometry of depth D (rows per lane = D).
 template <int D> struct ChunkShape {
-  static constexpr int BY = D <= 3 ? 16 : (D == 4 ? 12 : 8);
+  static constexpr int BY = D <= 3 ? 16 : (D <= 5 ? 12 : 8);
 };
 
 template <int D, int BY>
@@ -144,7 +145,7 @@ __device__ __forceinline__ void chunk_tile(const ChunkArgs& a, int tile, ChunkLd
     if constexpr (!REPAIR) {
       const int tb = t % kEnergyFrames;
       if (tb == kEnergyFrames - 1 || t == T - 1) {
-        // Strided flush into E[t][i0 + u], u < nup.
+        // Strided flush into E[t][i0 + u], u < nup (slots are D apart).
         __syncthreads();
         const int t0 = t - tb;
         for (int i = threadIdx.x; i < (tb + 1) * D; i += blockDim.x) {
@@ -212,6 +213,10 @@ __global__ void k_fill_int(int* p, int n, int v) {
   if (i < n) p[i] = v;
 }
 
+// Chunks of up to d updates run on the tile geometry of depth D = d + 1:
+// with p loaded, the first divergence already needs p0 from the row above
+// and p1 from the column left of the tile, so the top and left halos need
+// d + 1 rows/columns (the bottom and right ones d).
 template <int D>
 int launch_chunk(ChunkArgs a, hipStream_t s) {
   constexpr int BY = ChunkShape<D>::BY;
@@ -240,12 +245,12 @@ int launch_chunk(ChunkArgs a, hipStream_t s) {
 
 int launch_chunk_depth(int d, const ChunkArgs& a, hipStream_t s) {
   switch (d) {
-    case 1: return launch_chunk<1>(a, s);
-    case 2: return launch_chunk<2>(a, s);
-    case 3: return launch_chunk<3>(a, s);
-    case 4: return launch_chunk<4>(a, s);
-    case 5: return launch_chunk<5>(a, s);
-    case 6: return launch_chunk<6>(a, s);
+    case 1: return launch_chunk<2>(a, s);
+    case 2: return launch_chunk<3>(a, s);
+    case 3: return launch_chunk<4>(a, s);
+    case 4: return launch_chunk<5>(a, s);
+    case 5: return launch_chunk<6>(a, s);
+    case 6: return launch_chunk<7>(a, s);
   }
   return fail(SCIGAP_EUNSUPPORTED, "chunk depth %d not instantiated", d);
 }
