@@ -154,8 +154,10 @@ typedef struct scigap_params {
   int engine;          /* fused-kernel engine for tv_iter_max = 5 (0 =
                           default): 1 two LDS exchanges per TV update,
                           2 one exchange per update, 3/4 as 1/2 with
-                          interior and border tiles in separate launches;
-                          a tuning knob with no effect on results        */
+                          interior and border tiles in separate launches,
+                          5 as 3 with the next frame prefetched straight
+                          into LDS; a tuning knob with no effect on
+                          results                                        */
 } scigap_params;
 
 /* Fill *p with the reference defaults for a GAP solve of an HxWxT problem:

@@ -168,14 +168,42 @@ __device__ __forceinline__ void commit_err(float err, double* dst, double* red64
 // the yb accumulator and the prefetched next frame (x, Phi); the TV input
 // img lives in LDS as a [tile row][128] float plane (the engine reads it on
 // every divergence).
-template <int D, int BY, int MR, bool X1 = false>
+template <int LH>
+struct DmaPlanes {  // next frame's x and Phi rows, [tile row][128] floats
+  float x[LH * 128];
+  float ph[LH * 128];
+};
+struct NoDmaPlanes {};
+
+template <int D, int BY, int MR, bool X1 = false, bool DMA = false>
 struct FusedLds {
   static constexpr int LH = TileGeom<D, BY, MR>::LH;
   std::conditional_t<X1, TileLdsX1<BY>, TileLds<BY>> eng;
   float2 img[LH * 64];
-  float red[kEnergyFrames * D];  // per-frame energy partials of this tile
+  float red[kEnergyFrames * BY * D];  // per-frame, per-wave energy partials
   double red64[BY];
+  std::conditional_t<DMA, DmaPlanes<LH>, NoDmaPlanes> dma;
 };
+
+// Direct-to-LDS prefetch of RY rows (RY even) of one plane for this wave:
+// each buffer_load_dwordx4 ... lds moves two rows (lanes 0-31: row j, lanes
+// 32-63: row j+1, four floats per lane) into `rows` ([row][128] floats,
+// wave-uniform address).  Interior tiles only (the rows lie in the image and
+// start 16-byte aligned when W % 4 == 0).
+template <int RY>
+__device__ __forceinline__ void dma_rows(const float* plane, int N, float* rows, int off0,
+                                         int W, int lane) {
+  static_assert(RY % 2 == 0, "two rows per transfer");
+  const __amdgpu_buffer_rsrc_t rs = plane_rsrc(plane, N);
+  // off0 is the offset of this lane's first column pair; the transfer needs
+  // the tile row start (lane 0's column) plus 4 floats per lane.
+  const int base = off0 - 2 * lane;
+  const unsigned vo = unsigned(base + (lane >> 5) * W + 4 * (lane & 31)) * 4u;
+#pragma unroll
+  for (int j = 0; j < RY; j += 2)
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+        rs, (__attribute__((address_space(3))) void*)(rows + j * 128), 16, vo, j * W * 4, 0, 0);
+}
 
 // Byte offset of pixel (j,k) of the lane's block for a buffer store, or an
 // offset past the end of any plane (the store is dropped) when the lane does
@@ -195,12 +223,15 @@ __device__ __forceinline__ void st_at(float* p, int N, unsigned byte_off, float 
 // batched persistent kernel).  Returns without work when the tile's class
 // (interior/border) differs from INSIDE.  X1 selects the one-barrier TV
 // engine (tv_run_x1).
-template <int D, int BY, int MR, int MODE, bool INSIDE, bool IQA, bool X1 = false>
+template <int D, int BY, int MR, int MODE, bool INSIDE, bool IQA, bool X1 = false,
+          bool DMA = false>
 __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
-                                           FusedLds<D, BY, MR, X1>& sh) {
+                                           FusedLds<D, BY, MR, X1, DMA>& sh) {
   using G = TileGeom<D, BY, MR>;
   constexpr int RY = G::RY;
   constexpr int NH = MR;
+  // Interior tiles may prefetch the next frame straight into LDS.
+  constexpr bool kDma = DMA && INSIDE && (RY % 2 == 0);
   auto& lds = sh.eng;
   float* const red = sh.red;
   double* const red64 = sh.red64;
@@ -237,7 +268,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
 #pragma unroll
     for (int j = 0; j < RY; ++j) yb[j][0] = yb[j][1] = 0.f;
   }
-  for (int i = threadIdx.x; i < kEnergyFrames * D; i += blockDim.x) red[i] = 0.f;
+  for (int i = threadIdx.x; i < kEnergyFrames * BY * D; i += blockDim.x) red[i] = 0.f;
   float err = 0.f;
 
   // Frame t+1's x and Phi are requested before frame t's TV so that their
@@ -245,8 +276,17 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
   float nx[RY][2], nph[RY][2];
   const float* phit = a.phi;
   const float* xt = a.x_in;
-  if constexpr (MODE != TV_ONLY) load_plane<RY, INSIDE>(phit, nph, lp, off0, H, W);
-  if (!from_y) load_plane<RY, INSIDE>(xt, nx, lp, off0, H, W);
+  float* dma_x = nullptr;
+  float* dma_ph = nullptr;
+  if constexpr (kDma) {
+    dma_x = sh.dma.x + lp.w * RY * 128;
+    dma_ph = sh.dma.ph + lp.w * RY * 128;
+    if constexpr (MODE != TV_ONLY) dma_rows<RY>(phit, N, dma_ph, off0, W, lp.lane);
+    if (!from_y) dma_rows<RY>(xt, N, dma_x, off0, W, lp.lane);
+  } else {
+    if constexpr (MODE != TV_ONLY) load_plane<RY, INSIDE>(phit, nph, lp, off0, H, W);
+    if (!from_y) load_plane<RY, INSIDE>(xt, nx, lp, off0, H, W);
+  }
 
   float* xo = a.x_out;
   const float* trt = a.truth;
@@ -258,6 +298,23 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
   const bool owns_any = tp.own_r != 0;  // wave-uniform: halo waves own nothing
   for (int t = 0; t < T; ++t) {
     float ph[RY][2];
+    if constexpr (kDma) {
+      // This wave's rows of frame t arrived by DMA (issued one frame ago).
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+#pragma unroll
+      for (int j = 0; j < RY; ++j) {
+        if constexpr (MODE != TV_ONLY) {
+          const float2 f = reinterpret_cast<const float2*>(dma_ph + j * 128)[lp.lane];
+          nph[j][0] = f.x;
+          nph[j][1] = f.y;
+        }
+        if (!from_y) {
+          const float2 v = reinterpret_cast<const float2*>(dma_x + j * 128)[lp.lane];
+          nx[j][0] = v.x;
+          nx[j][1] = v.y;
+        }
+      }
+    }
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
       if constexpr (MODE == TV_ONLY) {
@@ -270,8 +327,16 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
       }
     }
     if (t + 1 < T) {
-      if constexpr (MODE != TV_ONLY) load_plane<RY, INSIDE>(phit + N, nph, lp, off0, H, W);
-      if (!from_y) load_plane<RY, INSIDE>(xt + N, nx, lp, off0, H, W);
+      if constexpr (kDma) {
+        // The planes were read above; let those reads finish before the
+        // next frame's rows overwrite them.
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+        if constexpr (MODE != TV_ONLY) dma_rows<RY>(phit + N, N, dma_ph, off0, W, lp.lane);
+        if (!from_y) dma_rows<RY>(xt + N, N, dma_x, off0, W, lp.lane);
+      } else {
+        if constexpr (MODE != TV_ONLY) load_plane<RY, INSIDE>(phit + N, nph, lp, off0, H, W);
+        if (!from_y) load_plane<RY, INSIDE>(xt + N, nx, lp, off0, H, W);
+      }
     }
 
     float out[RY][2];
@@ -317,9 +382,9 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
       }
     }
     const int tb = t % kEnergyFrames;
-    energy_add<D, BY, MR, NH>(acc, tp.lane_owner, red + tb * D, lp);
+    energy_add<D, BY, MR, NH>(acc, tp.lane_owner, red + tb * BY * D, lp);
     if (tb == kEnergyFrames - 1 || t == T - 1) {
-      energy_flush<D>(red, tb + 1, en - size_t(tb) * D);
+      energy_flush<D, BY>(red, tb + 1, en - size_t(tb) * D);
     }
     phit += N;
     xt += N;
@@ -359,15 +424,16 @@ __host__ __device__ inline bool tile_inside(int tile, int tiles_x, int H, int W)
 // instantiations (the interior one has no masks or bounds checks).  PART
 // restricts a launch to one class (1: interior, 2: border; 0: both), so that
 // the interior kernel is allocated registers for its own body only.
-template <int D, int BY, int MR, int MODE, bool IQA, bool X1 = false, int PART = 0>
+template <int D, int BY, int MR, int MODE, bool IQA, bool X1 = false, int PART = 0,
+          bool DMA = false>
 __global__ void __launch_bounds__(64 * BY) k_fused(FusedArgs a) {
-  __shared__ FusedLds<D, BY, MR, X1> sh;
+  __shared__ FusedLds<D, BY, MR, X1, DMA> sh;
   const int tile = tile_of_block(blockIdx.x, a.per_xcd);
   if (tile >= a.ntiles) return;
   const bool inside = tile_inside<D, BY, MR>(tile, a.tiles_x, a.H, a.W);
   if constexpr (PART != 2) {
     if (inside) {
-      fused_tile<D, BY, MR, MODE, true, IQA, X1>(a, tile, sh);
+      fused_tile<D, BY, MR, MODE, true, IQA, X1, DMA>(a, tile, sh);
       return;
     }
   }

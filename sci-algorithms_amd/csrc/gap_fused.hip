@@ -66,9 +66,9 @@ unsigned set_tiles(FusedArgs& a) {
   return unsigned(a.per_xcd * 8);
 }
 
-template <int D, int BY, int MODE, bool IQA, bool X1, int PART>
+template <int D, int BY, int MODE, bool IQA, bool X1, int PART, bool DMA = false>
 void launch_kernel(const FusedArgs& a, unsigned grid, hipStream_t s) {
-  k_fused<D, BY, 1, MODE, IQA, X1, PART><<<grid, 64 * BY, 0, s>>>(a);
+  k_fused<D, BY, 1, MODE, IQA, X1, PART, DMA><<<grid, 64 * BY, 0, s>>>(a);
 }
 
 // Engines of the fused kernel (scigap_params.engine):
@@ -76,6 +76,8 @@ void launch_kernel(const FusedArgs& a, unsigned grid, hipStream_t s) {
 //   2  one exchange per update (tv_run_x1), one launch
 //   3  as 1, interior and border tiles in separate launches
 //   4  as 2, separate launches
+//   5  as 3, and interior tiles prefetch the next frame straight into LDS
+//      (buffer_load ... lds) instead of registers (needs W % 4 == 0, else 3)
 // The alternatives exist for tv_iter_max = 5 (D = 4); other depths run 1.
 // engine 0 picks the default engine for the depth.
 constexpr int kDefaultEngine = 1;
@@ -89,8 +91,11 @@ int launch_main(const FusedArgs& a0, hipStream_t s, int engine) {
     ProfScope ps(MODE == TV_ONLY ? "tv_fused" : "fused_iteration", s);
     if constexpr (D == 4) {
       constexpr bool x1_fits = sizeof(FusedLds<D, BY, 1, true>) <= 160 * 1024;
-      const bool split = engine == 3 || engine == 4;
-      if (x1_fits && (engine == 2 || engine == 4)) {
+      const bool split = engine == 3 || engine == 4 || engine == 5;
+      if (engine == 5 && a.W % 4 == 0) {
+        launch_kernel<D, BY, MODE, IQA, false, 1, true>(a, grid, s);
+        launch_kernel<D, BY, MODE, IQA, false, 2>(a, grid, s);
+      } else if (x1_fits && (engine == 2 || engine == 4)) {
         if constexpr (x1_fits) {
           if (split) {
             launch_kernel<D, BY, MODE, IQA, true, 1>(a, grid, s);
@@ -223,7 +228,7 @@ int check_params(const scigap_params* p) {
   if (!(p->tv_weight > 0.f)) return fail(SCIGAP_EINVAL, "tv_weight must be > 0");
   if (p->tile_rows != 0 && p->tile_rows != 8 && p->tile_rows != 12 && p->tile_rows != 16)
     return fail(SCIGAP_EINVAL, "tile_rows must be 0, 8, 12 or 16");
-  if (p->engine < 0 || p->engine > 4) return fail(SCIGAP_EINVAL, "engine must be in [0, 4]");
+  if (p->engine < 0 || p->engine > 5) return fail(SCIGAP_EINVAL, "engine must be in [0, 5]");
   if (p->H * p->W > (int64_t(1) << 30) - 8)
     return fail(SCIGAP_EINVAL, "H*W must be <= 2^30 - 8 (buffer-addressed planes)");
   return SCIGAP_OK;

@@ -341,17 +341,21 @@ __device__ __forceinline__ void tv_run_x1(const float2* img, float (&out)[RY][2]
   }
 }
 
-// Frame energies: each wave reduces its lanes' partials (DPP) and adds them
-// into an LDS slot of the frame with an LDS float atomic, so frames need no
-// barrier; energy_flush moves a block of frames to the global fp64
-// accumulators (one barrier and T*D atomics per workgroup per block).
-// A lane contributes half h of its strip when those rows are interior rows of
-// the tile and its columns are interior columns.
+// Frame energies: each wave reduces its lanes' partials to wave-uniform sums
+// (DPP and lane reads) and lane 0 writes them to the wave's slots of the
+// frame, [frame % kEnergyFrames][wave][update] floats of LDS, so frames need
+// no barrier and no atomics; energy_flush sums the waves of a block of frames
+// in a fixed order and adds the totals to the global fp64 accumulators (one
+// barrier pair and T*D atomics per workgroup per block).  Halo waves hold no
+// interior rows and never write: their slots keep the zeros written at the
+// start of the tile.  A lane contributes half h of its strip when those rows
+// are interior rows of the tile and its columns are interior columns.
 constexpr int kEnergyFrames = 32;  // frames buffered in LDS between flushes
 
 template <int D, int BY, int MR, int NH>
 __device__ __forceinline__ void energy_add(const float (&acc)[NH * 8], bool lane_owner,
-                                           float* slot /* LDS, D floats */, const LanePos& lp) {
+                                           float* frame_slots /* LDS, BY*D floats */,
+                                           const LanePos& lp) {
   bool lo_in, hi_in;
   if (MR == 2) {
     lo_in = lp.w > 0;          // rows [0, D) of wave 0 are the top halo
@@ -360,6 +364,7 @@ __device__ __forceinline__ void energy_add(const float (&acc)[NH * 8], bool lane
     lo_in = hi_in = lp.w > 0 && lp.w < BY - 1;
   }
   if (!lo_in && !hi_in) return;  // wave-uniform: halo waves hold no interior rows
+  float s[D];
 #pragma unroll
   for (int u = 0; u < D; ++u) {
     float v = 0.f;
@@ -367,19 +372,24 @@ __device__ __forceinline__ void energy_add(const float (&acc)[NH * 8], bool lane
       v = lo_in ? acc[u] : 0.f;
       if (NH == 2) v += hi_in ? acc[8 + u] : 0.f;
     }
-    v = wave_sum_dpp(v);
-    if (lp.lane == 0) atomicAdd(slot + u, v);
+    s[u] = wave_sum_uniform(v);
   }
+  if (lp.lane == 0)
+#pragma unroll
+    for (int u = 0; u < D; ++u) frame_slots[lp.w * D + u] = s[u];
 }
 
-// Add frames [t0, t0+nt) of the LDS block to energy (fp64, [T][D]) and clear
-// the block.  All threads of the workgroup call it.
-template <int D>
-__device__ __forceinline__ void energy_flush(float* block, int nt, double* energy_t0) {
+// Add frames [t0, t0+nt) of the LDS block ([frame][wave][update]) to energy
+// (fp64, [T][D] from frame t0).  All threads of the workgroup call it.
+template <int D, int BY>
+__device__ __forceinline__ void energy_flush(const float* block, int nt, double* energy_t0) {
   __syncthreads();
   for (int i = threadIdx.x; i < nt * D; i += blockDim.x) {
-    atomicAdd(energy_t0 + i, double(block[i]));
-    block[i] = 0.f;
+    const int t = i / D, u = i - t * D;
+    float s = 0.f;
+#pragma unroll 4
+    for (int w = 0; w < BY; ++w) s += block[(t * BY + w) * D + u];
+    atomicAdd(energy_t0 + i, double(s));
   }
   __syncthreads();
 }
