@@ -15,6 +15,7 @@ the FP64 oracle.
 Sources restated (paths relative to the reference checkout):
 
 * ``PnP_SCI/python/utils.py:10-36``         -- ``A_``, ``At_``, ``psnr``
+* ``PnP_SCI/python/pnp_sci_algo.py:20-265``  -- ``gap_denoise_bayer`` (TV)
 * ``PnP_SCI/python/pnp_sci_algo.py:479-534`` -- ``admmdenoise_cacti``
 * ``PnP_SCI/python/pnp_sci_algo.py:536-706`` -- ``gap_denoise``
 * ``PnP_SCI/python/pnp_sci_algo.py:708-864`` -- ``admm_denoise``
@@ -38,6 +39,9 @@ How each function is pinned (the tests under ``tests/`` do the checking):
   with an identity denoiser is an exact projection (A x = y where Phi_sum > 0);
   the first accelerated step equals the plain step; ADMM with gamma=0 and
   an identity denoiser reproduces plain GAP.
+* ``gap_denoise_bayer``: equals four independent ``gap_denoise`` runs on the
+  2x2 sub-grids reassembled into the mosaic (the TV is per frame, so the
+  joint multichannel TV decouples); mosaic PSNRs equal the closed form.
 * ``psnr``: closed form (``utils.py:28-36``), 100 for a zero error.
 * ``compare_ssim``: golden values written by scikit-image 0.18.3.
 """
@@ -51,7 +55,7 @@ import numpy as np
 __all__ = [
     "A_", "At_", "psnr", "compare_psnr", "phi_sum", "tv_chambolle_nd",
     "denoise_tv_chambolle", "gap_denoise", "admm_denoise", "admm_denoise_clip",
-    "admmdenoise_cacti", "GAP_TV_rec", "ADMM_TV_rec", "compare_ssim",
+    "admmdenoise_cacti", "GAP_TV_rec", "ADMM_TV_rec", "compare_ssim", "gap_denoise_bayer",
 ]
 
 
@@ -325,6 +329,77 @@ def admm_denoise_clip(y, Phi_sum, A, At, _lambda=1, gamma=0.0, **kw):
     """joint_pnp_sci_algo.py:502-663: ADMM-TV with theta clipped to [0,1]."""
     return admm_denoise(y, Phi_sum, A, At, _lambda=_lambda, gamma=gamma,
                         clip=True, **kw)
+
+
+_BAYER = [[0, 0], [0, 1], [1, 0], [1, 1]]  # 2x2 sub-grids, pnp_sci_algo.py:100-102
+
+
+def gap_denoise_bayer(y_bayer, Phi_bayer, _lambda=1, accelerate=True, denoiser='tv',
+                      iter_max=50, noise_estimate=True, sigma=None, tv_weight=0.1,
+                      tv_iter_max=5, multichannel=True, x0_bayer=None, X_orig=None,
+                      model=None, show_iqa=True, _tv_eps=2.e-4):
+    """GAP-TV on a Bayer mosaic, pnp_sci_algo.py:20-265 with denoiser='tv'.
+
+    The mosaic is split into its four 2x2 sub-grids (:104-136); every outer
+    iteration applies the GAP update to each sub-grid (:150-157) and then one
+    multichannel TV over the nmask*4 sub-grid frames (:161-165); the result is
+    reassembled into the mosaic (:248-250).  Per-iteration PSNR is taken on
+    the mosaic (:227-231).  The reference allocates the sub-grid arrays in
+    float32 (:108-111); here they follow the input, so float64 inputs give
+    the FP64 oracle.  Returns ``(x_bayer, psnr_, ssim_, psnr_all)``.
+    """
+    _check_denoiser(denoiser)
+    sigma, iter_max = _stages(sigma, iter_max)
+    nrow, ncol, nmask = Phi_bayer.shape
+    dt = np.float32
+    yall = np.zeros([nrow // 2, ncol // 2, 4], dtype=dt)
+    Phiall = np.zeros([nrow // 2, ncol // 2, nmask, 4], dtype=dt)
+    Phi_sumall = np.zeros([nrow // 2, ncol // 2, 4], dtype=dt)
+    x0all = np.zeros([nrow // 2, ncol // 2, nmask, 4], dtype=dt)
+    if np.asarray(y_bayer).dtype == np.float64:
+        dt = np.float64
+        yall, Phiall, Phi_sumall, x0all = (a.astype(dt) for a in (yall, Phiall, Phi_sumall, x0all))
+    for ib, b in enumerate(_BAYER):
+        yall[..., ib] = y_bayer[b[0]::2, b[1]::2]
+        Phiall[..., ib] = Phi_bayer[b[0]::2, b[1]::2]
+        ps = np.sum(Phiall[..., ib], axis=2)
+        ps[ps == 0] = 1
+        Phi_sumall[..., ib] = ps
+        if x0_bayer is None:
+            x0all[..., ib] = At_(yall[..., ib], Phiall[..., ib])
+        else:
+            x0all[..., ib] = x0_bayer[b[0]::2, b[1]::2]
+    y1all = np.zeros_like(yall)
+    xall = x0all
+    x_bayer = np.zeros_like(Phi_bayer)
+    psnr_all = []
+    for idx, _nsig in enumerate(sigma):
+        for _ in range(iter_max[idx]):
+            for ib in range(4):
+                yb = A_(xall[..., ib], Phiall[..., ib])
+                if accelerate:
+                    y1all[..., ib] += (yall[..., ib] - yb)
+                    xall[..., ib] += _lambda * At_((y1all[..., ib] - yb) / Phi_sumall[..., ib],
+                                                   Phiall[..., ib])
+                else:
+                    xall[..., ib] += _lambda * At_((yall[..., ib] - yb) / Phi_sumall[..., ib],
+                                                   Phiall[..., ib])
+            xv = xall.reshape([nrow // 2, ncol // 2, nmask * 4])
+            xv = denoise_tv_chambolle(xv, tv_weight, eps=_tv_eps, n_iter_max=tv_iter_max,
+                                      multichannel=multichannel)
+            xall = xv.reshape([nrow // 2, ncol // 2, nmask, 4])
+            if show_iqa and X_orig is not None:
+                for ib, b in enumerate(_BAYER):
+                    x_bayer[b[0]::2, b[1]::2] = xall[..., ib]
+                psnr_all.append(compare_psnr(X_orig, x_bayer, data_range=1.))
+    for ib, b in enumerate(_BAYER):
+        x_bayer[b[0]::2, b[1]::2] = xall[..., ib]
+    psnr_, ssim_ = [], []
+    if X_orig is not None:
+        for m in range(nmask):
+            psnr_.append(compare_psnr(X_orig[:, :, m], x_bayer[:, :, m], data_range=1.))
+            ssim_.append(compare_ssim(X_orig[:, :, m], x_bayer[:, :, m], data_range=1.))
+    return x_bayer, psnr_, ssim_, psnr_all
 
 
 def admmdenoise_cacti(meas, mask, A, At, projmeth='admm', v0=None, orig=None,

@@ -5,7 +5,7 @@ compressive imaging: the ``denoiser='tv'`` path of PnP-SCI
 Public names mirror the reference modules:
 
 * ``pnp_sci.pnp_sci_algo``: gap_denoise, admm_denoise, admmdenoise_cacti,
-  GAP_TV_rec, ADMM_TV_rec
+  GAP_TV_rec, ADMM_TV_rec, gap_denoise_bayer
 * ``pnp_sci.utils``: A_, At_, psnr, make_operators
 * ``pnp_sci.tv``: denoise_tv_chambolle (scikit-image signature)
 
@@ -18,9 +18,10 @@ Beyond the reference:
 """
 
 from .pnp_sci_algo import (ADMM_TV_rec, GAP_TV_rec, admm_denoise, admmdenoise_cacti,
-                           gap_denoise, solve_device)
+                           gap_denoise, gap_denoise_bayer, solve_device)
 from .tv import denoise_tv_chambolle
 from .utils import A_, At_, make_operators, psnr
 
 __all__ = ["gap_denoise", "admm_denoise", "admmdenoise_cacti", "GAP_TV_rec", "ADMM_TV_rec",
+           "gap_denoise_bayer",
            "solve_device", "denoise_tv_chambolle", "A_", "At_", "psnr", "make_operators"]

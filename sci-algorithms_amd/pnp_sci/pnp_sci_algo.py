@@ -2,7 +2,7 @@
 
 Drop-in mirror of the TV path of ``PnP_SCI/python/pnp_sci_algo.py``:
 ``gap_denoise`` (:536-706), ``admm_denoise`` (:708-864),
-``admmdenoise_cacti`` (:479-534) and the legacy ``GAP_TV_rec`` /
+``gap_denoise_bayer`` (:20-265), ``admmdenoise_cacti`` (:479-534) and the legacy ``GAP_TV_rec`` /
 ``ADMM_TV_rec`` (:866-907), plus the clipped ADMM of
 ``joint_pnp_sci_algo.py:502-663`` as ``admm_denoise(..., clip=True)``.
 Same argument names, defaults, return tuples and error behaviour; every outer
@@ -31,7 +31,7 @@ from .device import hwt_to_thw, is_torch, thw_to_hwt, to_device, to_host_like
 from .metrics import psnr_frames, ssim_frames
 
 __all__ = ["gap_denoise", "admm_denoise", "admmdenoise_cacti", "GAP_TV_rec", "ADMM_TV_rec",
-           "recover_mask", "solve_device"]
+           "gap_denoise_bayer", "recover_mask", "solve_device"]
 
 _OTHER_DENOISERS = ("wavelet", "ffdnet", "fastdvdnet", "vnlnet")
 
@@ -236,6 +236,67 @@ def admm_denoise(y, Phi_sum, A, At, _lambda=1, gamma=0.01,
                 clip=clip, denoiser=denoiser, iter_max=iter_max, sigma=sigma,
                 tv_weight=tv_weight, tv_iter_max=tv_iter_max, multichannel=multichannel,
                 x0=x0, X_orig=X_orig, show_iqa=show_iqa)
+
+
+_BAYER = [[0, 0], [0, 1], [1, 0], [1, 1]]  # 2x2 sub-grids (pnp_sci_algo.py:100-102)
+
+
+def gap_denoise_bayer(y_bayer, Phi_bayer, _lambda=1, accelerate=True, denoiser='tv',
+                      iter_max=50, noise_estimate=True, sigma=None, tv_weight=0.1,
+                      tv_iter_max=5, multichannel=True, x0_bayer=None, X_orig=None,
+                      model=None, show_iqa=True):
+    """GAP-TV on a Bayer-mosaic measurement (pnp_sci_algo.py:20-265).
+
+    The reference splits the mosaic into its four 2x2 sub-grids, applies the
+    GAP update to each and one multichannel TV to all nmask*4 sub-grid frames
+    per outer iteration.  The TV is per frame, so the four sub-grids are
+    independent GAP-TV solves: each runs as one fused solve here, and the
+    results are written back into the mosaic.  y_bayer (H,W), Phi_bayer
+    (H,W,nmask) with H and W even.  Returns (x_bayer, psnr_, ssim_, psnr_all),
+    psnr_all on the whole mosaic per iteration.
+    """
+    torch = _torch()
+    _check_denoiser(denoiser, multichannel)
+    sigma, iter_max = _stages(sigma, iter_max)
+    iters = int(sum(iter_max))
+    yd = to_device(y_bayer, "y_bayer")
+    phid = to_device(Phi_bayer, "Phi_bayer")
+    nrow, ncol, nmask = phid.shape
+    if nrow % 2 or ncol % 2 or tuple(yd.shape) != (nrow, ncol):
+        raise ValueError("y_bayer (H,W) and Phi_bayer (H,W,nmask) need equal, even H and W")
+    want_iqa = show_iqa and X_orig is not None
+    truth = to_device(X_orig, "X_orig") if X_orig is not None else None
+    x0d = to_device(x0_bayer, "x0_bayer") if x0_bayer is not None else None
+    L = _lib.lib()
+    x_bayer = torch.empty((nrow, ncol, nmask), dtype=torch.float32, device=phid.device)
+    sq_total = None
+    for b0, b1 in _BAYER:
+        y_c = yd[b0::2, b1::2].contiguous()
+        phi_c = hwt_to_thw(phid[b0::2, b1::2].contiguous())
+        T, H, W = phi_c.shape
+        ps_c = torch.empty((H, W), dtype=torch.float32, device=phid.device)
+        _lib.check(L.scigap_phi_sum(phi_c.data_ptr(), ps_c.data_ptr(), H, W, T,
+                                    _lib.stream_ptr()))
+        x0_c = hwt_to_thw(x0d[b0::2, b1::2].contiguous()) if x0d is not None else None
+        tr_c = hwt_to_thw(truth[b0::2, b1::2].contiguous()) if want_iqa else None
+        x_c, sq, _ = solve_device(y_c, phi_c, ps_c, method="gap", iters=iters, _lambda=_lambda,
+                                  accelerate=accelerate, tv_weight=tv_weight,
+                                  tv_iter_max=tv_iter_max, x0=x0_c, truth=tr_c)
+        x_bayer[b0::2, b1::2] = thw_to_hwt(x_c)
+        if sq is not None:
+            sq_total = sq if sq_total is None else sq_total + sq
+    psnr_all = []
+    if want_iqa:
+        psnr_all = _psnr_from_sq(sq_total.cpu().tolist(), nrow * ncol * nmask)
+        stages = [None] * len(sigma) if noise_estimate else sigma
+        _print_progress("GAP", psnr_all, stages, iter_max)
+    psnr_, ssim_ = [], []
+    if X_orig is not None:
+        xt = hwt_to_thw(x_bayer)
+        tt = hwt_to_thw(truth)
+        psnr_ = psnr_frames(tt, xt)
+        ssim_ = ssim_frames(tt, xt) if min(nrow, ncol) >= 7 else []
+    return to_host_like(x_bayer, Phi_bayer), psnr_, ssim_, psnr_all
 
 
 def admmdenoise_cacti(meas, mask, A, At, projmeth='admm', v0=None, orig=None,

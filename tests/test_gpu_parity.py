@@ -404,3 +404,29 @@ def test_autotuner_choices_and_cache(gpu, tmp_path):
     t = tune.autotune_tv(img, n_iter_max=14)
     assert t["candidates"] == 6 and 1 <= t["tv_depth"] <= 6
     tune.clear()
+
+
+@pytest.mark.parametrize("accelerate", [True, False])
+def test_gap_denoise_bayer_matches_oracle(gpu, accelerate):
+    """gap_denoise_bayer (pnp_sci_algo.py:20-265): four sub-grid GAP-TV solves
+    reassembled into the mosaic, PSNR per iteration on the mosaic."""
+    import pnp_sci
+    from pnp_sci.synthetic import truth_video
+    g = np.random.default_rng(5)
+    H, W, T = 96, 132, 4
+    truth = truth_video(g, H, W, T)
+    mask = (g.random((H, W, T)) < 0.5).astype(np.float32)
+    y = (truth * mask).sum(2).astype(np.float32)
+    x, psnr_, ssim_, psnr_all = pnp_sci.gap_denoise_bayer(y, mask, accelerate=accelerate,
+                                                          iter_max=6, tv_weight=0.1,
+                                                          X_orig=truth)
+    wx, wp, ws, wall = O.gap_denoise_bayer(y.astype(np.float64), mask.astype(np.float64),
+                                           accelerate=accelerate, iter_max=6, tv_weight=0.1,
+                                           X_orig=truth.astype(np.float64))
+    assert x.shape == mask.shape and x.dtype == mask.dtype
+    assert rel(x, wx) <= 1e-4
+    assert np.max(np.abs(np.array(psnr_all) - np.array(wall))) <= 1e-3
+    assert np.max(np.abs(np.array(psnr_) - np.array(wp))) <= 1e-3
+    assert np.max(np.abs(np.array(ssim_) - np.array(ws))) <= 1e-4
+    with pytest.raises(ValueError):
+        pnp_sci.gap_denoise_bayer(y[:-1], mask[:-1], iter_max=1)

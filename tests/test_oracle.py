@@ -226,3 +226,28 @@ def test_ssim_matches_skimage_golden(golden_dir):
         assert abs(got - want) < 1e-12, (k, got, want)
     with pytest.raises(ValueError):
         O.compare_ssim(np.zeros((6, 9)), np.zeros((6, 9)))
+
+
+def test_bayer_gap_equals_four_subgrid_solves():
+    """gap_denoise_bayer (pnp_sci_algo.py:20-265) decouples into four GAP-TV
+    solves on the 2x2 sub-grids: the GAP update is per sub-grid and the
+    multichannel TV is per frame."""
+    g = np.random.default_rng(11)
+    H, W, T = 24, 30, 3
+    truth = g.random((H, W, T))
+    mask = (g.random((H, W, T)) < 0.5).astype(np.float64)
+    y = O.A_(truth, mask)
+    x, psnr_, ssim_, psnr_all = O.gap_denoise_bayer(y, mask, iter_max=4, tv_weight=0.1,
+                                                    X_orig=truth)
+    want = np.zeros_like(mask)
+    for b0, b1 in [[0, 0], [0, 1], [1, 0], [1, 1]]:
+        m = mask[b0::2, b1::2]
+        ps = m.sum(2)
+        ps[ps == 0] = 1
+        xc = O.gap_denoise(y[b0::2, b1::2], ps, lambda v: O.A_(v, m), lambda v: O.At_(v, m),
+                           iter_max=4, tv_weight=0.1, show_iqa=False)[0]
+        want[b0::2, b1::2] = xc
+    assert np.array_equal(x, want)
+    mse = np.mean((x - truth) ** 2)
+    assert abs(psnr_all[-1] - 10 * np.log10(1 / mse)) < 1e-9
+    assert len(psnr_all) == 4 and len(psnr_) == T and len(ssim_) == T
