@@ -156,8 +156,9 @@ typedef struct scigap_params {
                           2 one exchange per update, 3/4 as 1/2 with
                           interior and border tiles in separate launches,
                           5 as 3 with the next frame prefetched straight
-                          into LDS; a tuning knob with no effect on
-                          results                                        */
+                          into LDS, 6 as 1 with each tile's frames split
+                          over two workgroups; a tuning knob with no
+                          effect on results beyond rounding of sums      */
 } scigap_params;
 
 /* Fill *p with the reference defaults for a GAP solve of an HxWxT problem:

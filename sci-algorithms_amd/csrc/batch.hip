@@ -191,7 +191,7 @@ struct BatchLayout {
 };
 
 int batch_tiles(int64_t H, int64_t W, const Geom& g, int* tiles_x) {
-  const int tx = int((W + g.ox - 1) / g.ox), ty = int((H + g.oy - 1) / g.oy);
+  const int tx = tiles_across(int(W), 128, g.hc, g.ox), ty = int((H + g.oy - 1) / g.oy);
   if (tiles_x) *tiles_x = tx;
   return tx * ty;
 }

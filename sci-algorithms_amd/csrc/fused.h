@@ -41,6 +41,8 @@ struct FusedArgs {
   float lam, gam_next, weight, tw;
   double eps;
   int mode, clip, from_y, write_b, read_b, epilogue;
+  int fsplit;           // > 1: each tile's frames are split over fsplit workgroups
+  float* yb_part;       // [fsplit][H*W] partial yb of each frame part (fsplit > 1)
   int tiles_x, ntiles, per_xcd;
 };
 
@@ -103,6 +105,64 @@ __device__ __forceinline__ void load_plane(const float* __restrict__ src, float 
   }
 }
 
+// Column tiling.  When the image is at least one load tile wide (and W is
+// even) the first tile starts at column 0 and owns LW-HC columns (no halo is
+// needed at the image edge: the missing neighbours are the zeros of the
+// reference's boundary), the last tile ends at column W and owns what is
+// left (at most LW-HC columns, with a left halo of at least HC), and the
+// tiles between own OX columns each.  2048 columns then take 17 tiles instead
+// of 18.  Narrower or odd-width images use OX-column tiles from column -HC.
+// Ownership boundaries stay even, so every lane owns both or neither of its
+// columns.  Host and device code share these functions.
+struct ColPlace {
+  int c0;           // first loaded column
+  int own_lo;       // owned columns [own_lo, own_hi)
+  int own_hi;
+  bool edge_right;  // ends at the image's last column (needs its gradient mask)
+};
+
+__host__ __device__ inline bool cols_aligned(int W, int LW) { return W >= LW && W % 2 == 0; }
+
+__host__ __device__ inline int tiles_across(int W, int LW, int HC, int OX) {
+  if (cols_aligned(W, LW)) {
+    const int rest = W - 2 * (LW - HC);
+    return 2 + (rest > 0 ? (rest + OX - 1) / OX : 0);
+  }
+  return (W + OX - 1) / OX;
+}
+
+__host__ __device__ inline ColPlace col_place(int tx, int tiles_x, int W, int LW, int HC, int OX) {
+  ColPlace c;
+  if (cols_aligned(W, LW)) {
+    if (tx == 0) {
+      c.c0 = 0;
+      c.own_lo = 0;
+      c.own_hi = LW - HC;
+    } else if (tx == tiles_x - 1) {
+      c.c0 = W - LW;
+      c.own_lo = (LW - HC) + (tx - 1) * OX;
+      c.own_hi = W;
+    } else {
+      c.own_lo = (LW - HC) + (tx - 1) * OX;
+      c.c0 = c.own_lo - HC;
+      c.own_hi = c.own_lo + OX;
+    }
+    c.edge_right = tx == tiles_x - 1;
+  } else {
+    c.c0 = tx * OX - HC;
+    c.own_lo = c.c0 + HC;
+    c.own_hi = c.c0 + LW - HC;
+    c.edge_right = false;
+  }
+  return c;
+}
+
+template <int D, int BY, int MR>
+__host__ __device__ inline int tiles_x_of(int W) {
+  using G = TileGeom<D, BY, MR>;
+  return tiles_across(W, G::LW, G::HC, G::OX);
+}
+
 // Tile placement shared by the main and repair passes.
 template <int D, int BY, int MR>
 struct TilePlace {
@@ -115,14 +175,16 @@ struct TilePlace {
   unsigned own_r;  // bit j: row0+j is an interior row inside the image
   __device__ TilePlace(int tile, int tiles_x, int H, int W) {
     const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
-    const int c0 = tx * G::OX - G::HC, r0 = ty * G::OY - D;
+    const ColPlace cp = col_place(tx, tiles_x, W, G::LW, G::HC, G::OX);
+    const int c0 = cp.c0, r0 = ty * G::OY - D;
     lp.w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     lp.lane = threadIdx.x & 63;
     lp.row0 = r0 + lp.w * G::RY;
     lp.col = c0 + 2 * lp.lane;
     off0 = lp.row0 * W + lp.col;
-    inside = c0 >= 0 && r0 >= 0 && c0 + G::LW <= W && r0 + G::LH <= H && (W % 2 == 0);
-    lane_owner = lp.lane >= G::HC / 2 && lp.lane < 64 - G::HC / 2;
+    inside = c0 >= 0 && r0 >= 0 && c0 + G::LW <= W && r0 + G::LH <= H && (W % 2 == 0) &&
+             !cp.edge_right;
+    lane_owner = lp.col >= cp.own_lo && lp.col < cp.own_hi;
     own_c[0] = lane_owner && lp.col < W;
     own_c[1] = lane_owner && lp.col + 1 < W;
     own_r = 0;
@@ -226,7 +288,7 @@ __device__ __forceinline__ void st_at(float* p, int N, unsigned byte_off, float 
 template <int D, int BY, int MR, int MODE, bool INSIDE, bool IQA, bool X1 = false,
           bool DMA = false>
 __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
-                                           FusedLds<D, BY, MR, X1, DMA>& sh) {
+                                           FusedLds<D, BY, MR, X1, DMA>& sh, int part = 0) {
   using G = TileGeom<D, BY, MR>;
   constexpr int RY = G::RY;
   constexpr int NH = MR;
@@ -235,8 +297,12 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
   auto& lds = sh.eng;
   float* const red = sh.red;
   double* const red64 = sh.red64;
-  const int H = a.H, W = a.W, T = a.T;
+  const int H = a.H, W = a.W;
   const int N = H * W;
+  // Frames [t_lo, t_lo + T) of this work item (all frames unless split).
+  const int nparts = a.fsplit > 1 ? a.fsplit : 1;
+  const int t_lo = (a.T * part) / nparts;
+  const int T = (a.T * (part + 1)) / nparts - t_lo;
   const TilePlace<D, BY, MR> tp(tile, a.tiles_x, H, W);
   if (tp.inside != INSIDE) return;
   const LanePos& lp = tp.lp;
@@ -274,8 +340,8 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
   // Frame t+1's x and Phi are requested before frame t's TV so that their
   // latency hides behind the stencil arithmetic.
   float nx[RY][2], nph[RY][2];
-  const float* phit = a.phi;
-  const float* xt = a.x_in;
+  const float* phit = a.phi + size_t(t_lo) * N;
+  const float* xt = a.x_in + size_t(t_lo) * N;
   float* dma_x = nullptr;
   float* dma_ph = nullptr;
   if constexpr (kDma) {
@@ -288,10 +354,10 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
     if (!from_y) load_plane<RY, INSIDE>(xt, nx, lp, off0, H, W);
   }
 
-  float* xo = a.x_out;
-  const float* trt = a.truth;
-  float* bt = a.b;
-  double* en = a.energy;
+  float* xo = a.x_out + size_t(t_lo) * N;
+  const float* trt = a.truth ? a.truth + size_t(t_lo) * N : nullptr;
+  float* bt = a.b ? a.b + size_t(t_lo) * N : nullptr;
+  double* en = a.energy + size_t(t_lo) * D;
   // Every lane reads back only the LDS slots it wrote itself (the one-barrier
   // engine synchronises before it reads a neighbour's rows).
   float2* const img = sh.img + sl0;
@@ -403,6 +469,11 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
         if (!tp.own_c[k]) continue;
         const unsigned o = unsigned(off0 + j * W + k);
         const float v = yb[j][k];
+        if (nparts > 1) {
+          // the parts are summed (and the epilogue run) by k_part_epilogue
+          st1(a.yb_part + size_t(part) * N, N, o, v);
+          continue;
+        }
         if (a.yb) st1(a.yb, N, o, v);
         if (a.epilogue) epilogue_pixel<MODE>(a, o, v);
       }
@@ -411,13 +482,29 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
   if constexpr (IQA) commit_err<BY>(err, a.err, red64, lp.w, lp.lane);
 }
 
+// Sum of the frame parts' yb and the residual epilogue (fsplit > 1): the
+// frame loop of one tile is split over workgroups to even out the last
+// round of workgroups on the CUs; their partial yb planes meet here.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_part_epilogue(FusedArgs a) {
+  const int N = a.H * a.W;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    float yb = a.yb_part[i];
+    for (int f = 1; f < a.fsplit; ++f) yb += a.yb_part[size_t(f) * N + i];
+    if (a.yb) a.yb[i] = yb;
+    if (a.epilogue) epilogue_pixel<MODE>(a, unsigned(i), yb);
+  }
+}
+
 // Whether a tile's load region lies entirely in the image (mirrors TilePlace).
 template <int D, int BY, int MR>
 __host__ __device__ inline bool tile_inside(int tile, int tiles_x, int H, int W) {
   using G = TileGeom<D, BY, MR>;
   const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
-  const int c0 = tx * G::OX - G::HC, r0 = ty * G::OY - D;
-  return c0 >= 0 && r0 >= 0 && c0 + G::LW <= W && r0 + G::LH <= H && (W % 2 == 0);
+  const ColPlace cp = col_place(tx, tiles_x, W, G::LW, G::HC, G::OX);
+  const int c0 = cp.c0, r0 = ty * G::OY - D;
+  return c0 >= 0 && r0 >= 0 && c0 + G::LW <= W && r0 + G::LH <= H && (W % 2 == 0) &&
+         !cp.edge_right;
 }
 
 // One launch covers interior and border tiles; the two bodies are separate
@@ -428,17 +515,21 @@ template <int D, int BY, int MR, int MODE, bool IQA, bool X1 = false, int PART =
           bool DMA = false>
 __global__ void __launch_bounds__(64 * BY) k_fused(FusedArgs a) {
   __shared__ FusedLds<D, BY, MR, X1, DMA> sh;
-  const int tile = tile_of_block(blockIdx.x, a.per_xcd);
-  if (tile >= a.ntiles) return;
+  // Work item -> (tile, frame part); the parts of a tile are adjacent, so
+  // they run on the same XCD.
+  const int item = tile_of_block(blockIdx.x, a.per_xcd);
+  if (item >= a.ntiles) return;
+  const int nparts = a.fsplit > 1 ? a.fsplit : 1;
+  const int tile = item / nparts, part = item - tile * nparts;
   const bool inside = tile_inside<D, BY, MR>(tile, a.tiles_x, a.H, a.W);
   if constexpr (PART != 2) {
     if (inside) {
-      fused_tile<D, BY, MR, MODE, true, IQA, X1, DMA>(a, tile, sh);
+      fused_tile<D, BY, MR, MODE, true, IQA, X1, DMA>(a, tile, sh, part);
       return;
     }
   }
   if constexpr (PART != 1) {
-    if (!inside) fused_tile<D, BY, MR, MODE, false, IQA, X1>(a, tile, sh);
+    if (!inside) fused_tile<D, BY, MR, MODE, false, IQA, X1>(a, tile, sh, part);
   }
 }
 
@@ -783,6 +874,7 @@ struct SolveState {
   float* r[2] = {nullptr, nullptr};
   float* y1[2] = {nullptr, nullptr};
   float* yb = nullptr;       // (H,W)
+  float* yb_part = nullptr;  // [parts][H*W] partial yb of frame-split launches
   int* stop = nullptr;       // [T]
   int* nexit = nullptr;      // [iters]
   double* energy = nullptr;  // [iters][T][D]
@@ -831,6 +923,7 @@ inline FusedArgs iteration_args(const scigap_params& p, int k, int64_t H, int64_
   a.b = st.bbuf;
   a.energy = st.energy + size_t(k) * T * (D > 0 ? D : 1);
   a.yb = st.yb;
+  a.yb_part = st.yb_part;
   a.stop = st.stop;
   a.nexit = st.nexit + k;
   a.stats = stats ? stats + k : nullptr;

@@ -30,6 +30,8 @@
 // recomputes the frames that stopped at s < D from the unchanged inputs
 // (x, r and y1 are double-buffered) and corrects yb and the epilogue.
 
+#include <algorithm>
+
 #include "fused.h"
 
 namespace scigap {
@@ -59,7 +61,7 @@ template <int D> struct FusedShape {
 template <int D, int BY, int MR>
 unsigned set_tiles(FusedArgs& a) {
   using G = TileGeom<D, BY, MR>;
-  a.tiles_x = (a.W + G::OX - 1) / G::OX;
+  a.tiles_x = tiles_x_of<D, BY, MR>(a.W);
   const int tiles_y = (a.H + G::OY - 1) / G::OY;
   a.ntiles = a.tiles_x * tiles_y;
   a.per_xcd = (a.ntiles + 7) / 8;
@@ -78,16 +80,72 @@ void launch_kernel(const FusedArgs& a, unsigned grid, hipStream_t s) {
 //   4  as 2, separate launches
 //   5  as 3, and interior tiles prefetch the next frame straight into LDS
 //      (buffer_load ... lds) instead of registers (needs W % 4 == 0, else 3)
-// The alternatives exist for tv_iter_max = 5 (D = 4); other depths run 1.
-// engine 0 picks the default engine for the depth.
-constexpr int kDefaultEngine = 1;
+//   6  as 1, with each tile's frames split over F workgroups (their
+//      partial yb summed by k_part_epilogue); F is chosen from the tile
+//      count and the number of resident workgroups so that the last round
+//      of workgroups is full (large images) or the GPU fills at all (small
+//      images)
+// The alternatives 2-5 exist for tv_iter_max = 5 (D = 4).  engine 0 is 6.
+constexpr int kDefaultEngine = 6;
+constexpr int kMaxParts = 8;  // frame parts (yb_part planes in the workspace)
+
+// Workgroups of the fused kernel <D, BY, MODE, IQA> resident on the whole
+// device (queried once per instantiation; every device of a process is
+// assumed to be the same model).
+template <int D, int BY, int MODE, bool IQA>
+int resident_workgroups() {
+  static int cached = 0;
+  if (cached == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_fused<D, BY, 1, MODE, IQA, false, 0>, 64 * BY, 0) != hipSuccess ||
+        cus <= 0 || per_cu <= 0)
+      return 256;
+    cached = cus * per_cu;
+  }
+  return cached;
+}
+
+// Frame parts per tile: minimise rounds(F) * (frames per part + overhead),
+// with rounds(F) = ceil(tiles*F / resident workgroups).
+int pick_parts(int tiles, int T, int slots) {
+  int best = 1;
+  double best_cost = 1e300;
+  for (int F = 1; F <= std::min(T, kMaxParts); ++F) {
+    const double rounds = double((int64_t(tiles) * F + slots - 1) / slots);
+    const double cost = rounds * (double((T + F - 1) / F) + 0.25);
+    if (cost < best_cost - 1e-9) {
+      best_cost = cost;
+      best = F;
+    }
+  }
+  return best;
+}
 
 template <int D, int BY, int MODE, bool IQA>
 int launch_main(const FusedArgs& a0, hipStream_t s, int engine) {
   FusedArgs a = a0;
   const unsigned grid = set_tiles<D, BY, 1>(a);
   if (engine == 0) engine = kDefaultEngine;
-  {
+  int parts = 1;
+  if (engine == 6 && (MODE == TV_ONLY || a.yb_part)) {
+    const int slots = resident_workgroups<D, BY, MODE, IQA>();
+    parts = pick_parts(a.ntiles, a.T, slots);
+  }
+  if (parts > 1) {
+    a.fsplit = parts;
+    a.ntiles *= parts;
+    a.per_xcd = (a.ntiles + 7) / 8;
+    ProfScope ps(MODE == TV_ONLY ? "tv_fused" : "fused_iteration", s);
+    launch_kernel<D, BY, MODE, IQA, false, 0>(a, unsigned(a.per_xcd * 8), s);
+    if constexpr (MODE != TV_ONLY) {
+      const int64_t N = int64_t(a.H) * a.W;
+      const unsigned g = unsigned(std::min<int64_t>((N + 255) / 256, 4096));
+      k_part_epilogue<MODE><<<g, 256, 0, s>>>(a);
+    }
+  } else {
     ProfScope ps(MODE == TV_ONLY ? "tv_fused" : "fused_iteration", s);
     if constexpr (D == 4) {
       constexpr bool x1_fits = sizeof(FusedLds<D, BY, 1, true>) <= 160 * 1024;
@@ -206,6 +264,7 @@ Workspace carve(const scigap_params* p, char* base) {
     for (int i = 0; i < 2; ++i) w.st.y1[i] = reinterpret_cast<float*>(take(N * 4));
   w.st.energy = reinterpret_cast<double*>(take(K * T * D * 8));
   w.st.yb = reinterpret_cast<float*>(take(N * 4));
+  w.st.yb_part = reinterpret_cast<float*>(take(std::min<size_t>(T, kMaxParts) * N * 4));
   w.st.stop = reinterpret_cast<int*>(take(T * 4));
   w.st.nexit = reinterpret_cast<int*>(take(K * 4));
   if (p->tv_iter_max - 1 > kMaxFused) {
@@ -228,7 +287,7 @@ int check_params(const scigap_params* p) {
   if (!(p->tv_weight > 0.f)) return fail(SCIGAP_EINVAL, "tv_weight must be > 0");
   if (p->tile_rows != 0 && p->tile_rows != 8 && p->tile_rows != 12 && p->tile_rows != 16)
     return fail(SCIGAP_EINVAL, "tile_rows must be 0, 8, 12 or 16");
-  if (p->engine < 0 || p->engine > 5) return fail(SCIGAP_EINVAL, "engine must be in [0, 5]");
+  if (p->engine < 0 || p->engine > 6) return fail(SCIGAP_EINVAL, "engine must be in [0, 6]");
   if (p->H * p->W > (int64_t(1) << 30) - 8)
     return fail(SCIGAP_EINVAL, "H*W must be <= 2^30 - 8 (buffer-addressed planes)");
   return SCIGAP_OK;
@@ -245,7 +304,8 @@ int unrolled_forward_iterations(int64_t H, int64_t W, int64_t T, int K, int D,
                                 const float* lambdas, const float* weights, const float* y,
                                 const float* phi, const float* S, float* xs, size_t xstride,
                                 float* y1s, size_t y1stride, float* x_out, float* r2,
-                                double* energy, int* stop, int* nexit, hipStream_t s) {
+                                double* energy, int* stop, int* nexit, float* yb_part,
+                                hipStream_t s) {
   const int64_t N = H * W;
   if (D < 1 || D > kMaxFused) return fail(SCIGAP_EUNSUPPORTED, "unrolled depth %d", D);
   if (int rc = fill_f64(energy, K * int(T) * D, 0.0, s)) return rc;
@@ -269,6 +329,7 @@ int unrolled_forward_iterations(int64_t H, int64_t W, int64_t T, int K, int D,
     a.energy = energy + size_t(k) * T * D;
     a.stop = stop;
     a.nexit = nexit + k;
+    a.yb_part = yb_part;
     a.H = int(H);
     a.W = int(W);
     a.T = int(T);

@@ -371,7 +371,8 @@ int unrolled_forward_iterations(int64_t H, int64_t W, int64_t T, int K, int D,
                                 const float* lambdas, const float* weights, const float* y,
                                 const float* phi, const float* S, float* xs, size_t xstride,
                                 float* y1s, size_t y1stride, float* x_out, float* r2,
-                                double* energy, int* stop, int* nexit, hipStream_t s);
+                                double* energy, int* stop, int* nexit, float* yb_part,
+                                hipStream_t s);
 
 namespace {
 
@@ -401,6 +402,7 @@ struct UWorkspace {
   double* energy = nullptr; // K*T*D
   int* stop = nullptr;      // T
   int* nexit = nullptr;     // K
+  float* yb_part = nullptr; // [min(T,8)][H*W] partial yb of frame-split launches
   float* phibar = nullptr;  // (T,H,W) dL/dphi, when logits
   float* ybar = nullptr;    // (H,W) dL/dy of one sample
   double* gl = nullptr;     // K, when the caller passes no grad_lambda
@@ -425,6 +427,7 @@ UWorkspace carve(const scigap_unrolled& u, char* base) {
   w.energy = reinterpret_cast<double*>(take(K * size_t(u.T) * size_t(D) * 8));
   w.stop = reinterpret_cast<int*>(take(size_t(u.T) * 4));
   w.nexit = reinterpret_cast<int*>(take(K * 4));
+  w.yb_part = reinterpret_cast<float*>(take(size_t(u.T < 8 ? u.T : 8) * N * 4));
   if (u.logits) w.phibar = reinterpret_cast<float*>(take(V * 4));
   w.ybar = reinterpret_cast<float*>(take(N * 4));
   w.gl = reinterpret_cast<double*>(take(K * 8));
@@ -535,8 +538,8 @@ int scigap_unrolled_forward(const scigap_unrolled* u, const float* lambdas, cons
     if (int rc = scigap_adjoint(yb, phi, xs, u->H, u->W, u->T, s)) return rc;  // x_0 = At(y)
     if (int rc = unrolled_forward_iterations(u->H, u->W, u->T, u->iters, u->tv_iter_max - 1,
                                              lambdas, weights, yb, phi, w.S, xs, cl.xstride,
-                                             y1s, cl.y1stride, x_out + size_t(b) * V, w.r2, w.energy, w.stop,
-                                             w.nexit, s))
+                                             y1s, cl.y1stride, x_out + size_t(b) * V, w.r2,
+                                             w.energy, w.stop, w.nexit, w.yb_part, s))
       return rc;
   }
   return SCIGAP_OK;

@@ -160,10 +160,11 @@ int scigap_autotune_solve(const scigap_params* p, const float* y, const float* p
   std::vector<std::pair<int, int>> cands;
   if (p->tv_iter_max == 5) {
     for (int rows : {12, 8, 16})
-      for (int eng : {1, 2, 3, 4, 5}) cands.emplace_back(rows, eng);
+      for (int eng : {1, 2, 3, 4, 5, 6}) cands.emplace_back(rows, eng);
   } else {
     cands.emplace_back(0, 1);
     cands.emplace_back(0, 3);
+    cands.emplace_back(0, 6);
   }
   Entry best;
   double def_ms = -1.0;

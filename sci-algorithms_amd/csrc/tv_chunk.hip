@@ -224,7 +224,7 @@ template <int D>
 int launch_chunk(ChunkArgs a, hipStream_t s) {
   constexpr int BY = ChunkShape<D>::BY;
   using G = TileGeom<D, BY, 1>;
-  a.tiles_x = (a.W + G::OX - 1) / G::OX;
+  a.tiles_x = tiles_x_of<D, BY, 1>(a.W);
   a.ntiles = a.tiles_x * ((a.H + G::OY - 1) / G::OY);
   a.per_xcd = (a.ntiles + 7) / 8;
   const unsigned grid = unsigned(a.per_xcd * 8);

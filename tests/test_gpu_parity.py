@@ -335,7 +335,7 @@ def test_engines_and_tile_heights_agree(gpu, method):
     L = gpu
     outs = {}
     for rows in (8, 12, 16):
-        for eng in (1, 2, 3, 4, 5):
+        for eng in (1, 2, 3, 4, 5, 6):
             p = _lib.Params()
             L.scigap_default_params(p, 300, 520, 6)
             p.method = _lib.METHOD_ADMM if method == "admm" else _lib.METHOD_GAP
@@ -396,7 +396,7 @@ def test_autotuner_choices_and_cache(gpu, tmp_path):
     y, phi, ps = to_device(pr.y), to_device(pr.mask_thw), to_device(pr.phi_sum)
     before, _, _ = solve_device(y, phi, ps, iters=5)
     r = tune.autotune_solve(y, phi, ps, iters=2)
-    assert r["candidates"] == 15 and r["tile_rows"] in (8, 12, 16) and 1 <= r["engine"] <= 5
+    assert r["candidates"] == 18 and r["tile_rows"] in (8, 12, 16) and 1 <= r["engine"] <= 6
     assert r["best_ms"] <= r["default_ms"]
     after, _, _ = solve_device(y, phi, ps, iters=5)      # uses the tuned choice
     assert rel(after.cpu().numpy(), before.cpu().numpy()) <= 1e-6
