@@ -99,7 +99,12 @@ Key solve_key(const scigap_params& p) {
 template <class F>
 double time_on_stream(F&& fn, hipStream_t s, int reps, int* rc) {
   hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+  if (hipEventCreate(&e0) != hipSuccess) {
+    *rc = fail(SCIGAP_EHIP, "hipEventCreate failed");
+    return -1.0;
+  }
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
     *rc = fail(SCIGAP_EHIP, "hipEventCreate failed");
     return -1.0;
   }
