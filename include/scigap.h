@@ -149,15 +149,17 @@ typedef struct scigap_params {
   float gamma_decay;   /* per-iteration factor on gamma (1: none;
                           0.998 in ADMM_TV_rec, pnp_sci_algo.py:899)     */
   int tile_rows;       /* fused-kernel workgroup height in waves for
-                          tv_iter_max = 5: 8, 12 or 16 (0 = 12); a tuning
+                          tv_iter_max = 5: 8, 12 or 16 (0 = 8); a tuning
                           knob with no effect on results                 */
   int engine;          /* fused-kernel engine for tv_iter_max = 5 (0 =
-                          default): 1 two LDS exchanges per TV update,
+                          6): 1 two LDS exchanges per TV update,
                           2 one exchange per update, 3/4 as 1/2 with
                           interior and border tiles in separate launches,
                           5 as 3 with the next frame prefetched straight
                           into LDS, 6 as 1 with each tile's frames split
-                          over two workgroups; a tuning knob with no
+                          over up to 8 workgroups (as many as evens out
+                          the last round of workgroups on the CUs); a
+                          tuning knob with no
                           effect on results beyond rounding of sums      */
 } scigap_params;
 
@@ -209,8 +211,9 @@ typedef struct scigap_batch_options {
   int use_graph;   /* 1: capture the whole outer loop in a hipGraph on the
                       first run and replay it afterwards; 0: plain launches */
   int tile_rows;   /* waves per workgroup of the fused kernel for
-                      tv_iter_max = 5 (8, 12 or 16); 0 = 12               */
-  int grid;        /* persistent workgroups (0 = 256, one per CU)         */
+                      tv_iter_max = 5 (8, 12 or 16); 0 = 8                */
+  int grid;        /* persistent workgroups (0 = as many as the device
+                      holds at once)                                      */
   int order;       /* 0: tiles of the most frames first; 1: submission order */
 } scigap_batch_options;
 

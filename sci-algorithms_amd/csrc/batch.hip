@@ -166,7 +166,7 @@ Geom geom_of(int D, int BY) {
   return g;
 }
 
-int default_rows(int D) { return D < 4 ? 16 : (D == 4 ? 12 : 8); }
+int default_rows(int D) { return D < 4 ? 16 : 8; }
 
 int pick_rows(int D, const scigap_batch_options* opt) {
   if (D == 4 && opt && (opt->tile_rows == 8 || opt->tile_rows == 12 || opt->tile_rows == 16))
@@ -279,15 +279,33 @@ struct scigap_batch_plan {
   std::vector<scigap_batch_item> items;
   BatchLayout L;
   int n = 0, D = 0, BY = 16, KF = 0, mode = 0;
-  unsigned grid = 256;
+  unsigned grid = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
 
 namespace {
 
+// Workgroups of the persistent kernel that the device holds at once.
+template <int D, int BY, int MODE>
+unsigned resident_batch_grid() {
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_batch<D, BY, 1, MODE>, 64 * BY,
+                                                   0) != hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 256;
+  return unsigned(std::max(per_cu, 1) * std::max(cus, 1));
+}
+
+// Launches every fused iteration of the plan; with P->grid == 0 it only
+// sizes the persistent grid (no launch), which plan creation does once.
 template <int D, int BY, int MODE>
 int enqueue_iters(scigap_batch_plan* P, hipStream_t s) {
+  if (P->grid == 0) {
+    P->grid = resident_batch_grid<D, BY, MODE>();
+    return SCIGAP_OK;
+  }
   const BatchLayout& L = P->L;
   const int n = P->n;
   for (int k = 0; k < P->KF; ++k) {
@@ -383,7 +401,13 @@ int scigap_batch_plan_create(scigap_batch_plan** out, const scigap_params* p,
   P->BY = BY;
   P->mode = solver_mode(*p);
   P->KF = fused_iterations(*p);
-  P->grid = unsigned(P->opt.grid > 0 ? P->opt.grid : 256);
+  P->grid = unsigned(P->opt.grid > 0 ? P->opt.grid : 0);
+  if (P->grid == 0) {
+    if (int rc = enqueue_fused(P, nullptr)) {  // sizes the grid, launches nothing
+      delete P;
+      return rc;
+    }
+  }
 
   // Per-iteration launch arguments of every solve.
   const Geom g = geom_of(D, BY);

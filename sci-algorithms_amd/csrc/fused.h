@@ -339,7 +339,9 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
 
   // Frame t+1's x and Phi are requested before frame t's TV so that their
   // latency hides behind the stencil arithmetic.
-  float nx[RY][2], nph[RY][2];
+  // nx stays 0 when X_0 = At(y) (no previous x), so that img = q*Phi + nx
+  // serves both cases.
+  float nx[RY][2] = {}, nph[RY][2];
   const float* phit = a.phi + size_t(t_lo) * N;
   const float* xt = a.x_in + size_t(t_lo) * N;
   float* dma_x = nullptr;
@@ -388,8 +390,8 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
       } else {
 #pragma unroll
         for (int k = 0; k < 2; ++k) ph[j][k] = nph[j][k];
-        img[j * 64] = make_float2(from_y ? q[j][0] * ph[j][0] : fmaf(q[j][0], ph[j][0], nx[j][0]),
-                                  from_y ? q[j][1] * ph[j][1] : fmaf(q[j][1], ph[j][1], nx[j][1]));
+        img[j * 64] = make_float2(fmaf(q[j][0], ph[j][0], nx[j][0]),
+                                  fmaf(q[j][1], ph[j][1], nx[j][1]));
       }
     }
     if (t + 1 < T) {
@@ -614,7 +616,7 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
     const float* phit = a.phi + size_t(t) * N;
     const float* xt = a.x_in + size_t(t) * N;
     float* xo = a.x_out + size_t(t) * N;
-    float ph[RY][2], img[RY][2], xv[RY][2];
+    float ph[RY][2], img[RY][2], xv[RY][2] = {};
     if (tv_only) {
 #pragma unroll
       for (int j = 0; j < RY; ++j) ph[j][0] = ph[j][1] = 0.f;
@@ -627,7 +629,7 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
     for (int j = 0; j < RY; ++j) {
 #pragma unroll
       for (int k = 0; k < 2; ++k)
-        img[j][k] = a.from_y ? q[j][k] * ph[j][k] : fmaf(q[j][k], ph[j][k], xv[j][k]);
+        img[j][k] = fmaf(q[j][k], ph[j][k], xv[j][k]);  // xv = 0 when from_y
       simg[j * 64] = make_float2(img[j][0], img[j][1]);
     }
     float out[RY][2];

@@ -196,11 +196,12 @@ int launch_fused(const FusedArgs& a0, hipStream_t s, int tile_rows, int engine) 
   constexpr int BY = FusedShape<D>::BY, MR = FusedShape<D>::MR;
   int rc;
   if constexpr (D == 4) {
-    // 12 waves measured fastest at 2048x2048x20 (558 us against 697 us for
-    // 16 and 703 us for 8 per outer iteration).
-    rc = tile_rows == 8    ? launch_mode<4, 8>(a0, s, engine)
+    // 8 waves measured fastest at 2048x2048x20 (419 us against 509 us for
+    // 12 and 478 us for 16 per outer iteration): at about 110 VGPRs two
+    // 8-wave workgroups share a CU, so one hides the other's barriers.
+    rc = tile_rows == 12   ? launch_mode<4, 12>(a0, s, engine)
          : tile_rows == 16 ? launch_mode<4, 16>(a0, s, engine)
-                           : launch_mode<4, 12>(a0, s, engine);
+                           : launch_mode<4, 8>(a0, s, engine);
   } else {
     rc = launch_mode<D, BY>(a0, s, engine);
   }

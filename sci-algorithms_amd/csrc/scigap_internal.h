@@ -86,6 +86,11 @@ __device__ __forceinline__ float from_prev_lane(float v) {
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// Forces v to be computed at this point of the program.  Without it the
+// compiler may sink a running sum into a later wave-uniform branch and keep
+// every term of the sum live until then.
+__device__ __forceinline__ void compute_here(float& v) { asm volatile("" : "+v"(v)); }
+
 // Sum over the 64 lanes of a wavefront, result valid in every lane: DPP
 // within each 16-lane row (pairs, quads, half-row and row mirrors), then two
 // cross-row exchanges.

@@ -164,7 +164,7 @@ int scigap_autotune_solve(const scigap_params* p, const float* y, const float* p
   q.iters = iters;
   std::vector<std::pair<int, int>> cands;
   if (p->tv_iter_max == 5) {
-    for (int rows : {12, 8, 16})
+    for (int rows : {8, 12, 16})
       for (int eng : {1, 2, 3, 4, 5, 6}) cands.emplace_back(rows, eng);
   } else {
     cands.emplace_back(0, 1);

@@ -118,7 +118,10 @@ __device__ __forceinline__ void tv_div(const float2* img, float (&out)[RY][2],
   }
   if (dacc)
 #pragma unroll
-    for (int h = 0; h < NH; ++h) dacc[h * 8] += s[h];
+    for (int h = 0; h < NH; ++h) {
+      dacc[h * 8] += s[h];
+      compute_here(dacc[h * 8]);
+    }
 }
 
 // One dual update p <- (p - tau g)/(1 + tw |g|) from the current `out`;
@@ -166,7 +169,10 @@ __device__ __forceinline__ void tv_grad(const float (&out)[RY][2], float (&p0)[R
   }
   if (gacc)
 #pragma unroll
-    for (int h = 0; h < NH; ++h) gacc[h * 8] = fmaf(weight, s[h], gacc[h * 8]);
+    for (int h = 0; h < NH; ++h) {
+      gacc[h * 8] = fmaf(weight, s[h], gacc[h * 8]);
+      compute_here(gacc[h * 8]);
+    }
 }
 
 // Run `nup` (<= D) updates starting from p = 0 (out = img).  `img` points at
@@ -257,7 +263,10 @@ __device__ __forceinline__ void tv_grad_regs(const float (&out)[RY][2], float2 b
   }
   if (gacc)
 #pragma unroll
-    for (int h = 0; h < NH; ++h) gacc[h * 8] = fmaf(weight, s[h], gacc[h * 8]);
+    for (int h = 0; h < NH; ++h) {
+      gacc[h * 8] = fmaf(weight, s[h], gacc[h * 8]);
+      compute_here(gacc[h * 8]);
+    }
 }
 
 // out = img + d(p) with the p0 row above the strip in `above`.
@@ -282,7 +291,10 @@ __device__ __forceinline__ void tv_div_regs(const float2* img, float (&out)[RY][
   }
   if (dacc)
 #pragma unroll
-    for (int h = 0; h < NH; ++h) dacc[h * 8] += s[h];
+    for (int h = 0; h < NH; ++h) {
+      dacc[h * 8] += s[h];
+      compute_here(dacc[h * 8]);
+    }
 }
 
 // tv_run_from_zero with one barrier per update (plus one at entry, after
