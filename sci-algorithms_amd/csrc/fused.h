@@ -267,18 +267,24 @@ __device__ __forceinline__ void dma_rows(const float* plane, int N, float* rows,
         rs, (__attribute__((address_space(3))) void*)(rows + j * 128), 16, vo, j * W * 4, 0, 0);
 }
 
-// Byte offset of pixel (j,k) of the lane's block for a buffer store, or an
-// offset past the end of any plane (the store is dropped) when the lane does
-// not own the pixel.  Planes hold fewer than 2^30 - 8 floats (checked by the
-// host), so 0xFFFFFFF0 is always out of range.
-template <int D, int BY, int MR>
-__device__ __forceinline__ unsigned owned_offset(const TilePlace<D, BY, MR>& tp, int j, int k,
-                                                 int W) {
-  return tp.owns(j, k) ? unsigned(tp.off0 + j * W + k) * 4u : 0xFFFFFFF0u;
-}
-__device__ __forceinline__ void st_at(float* p, int N, unsigned byte_off, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), plane_rsrc(p, N), byte_off, 0,
-                                        0);
+// Stores of the lane's owned pixels (x and, for ADMM, b) of one frame.
+// Callers run it under `if (tp.own_c[0])` (per lane); owned rows are
+// wave-uniform.  The row offset is formed here, from a base that is made
+// opaque once per frame, so that the compiler does not keep RY per-row
+// offsets in registers across the frame loop.
+template <int D, int BY, int MR, bool INSIDE, int RY>
+__device__ __forceinline__ void store_owned(float* __restrict__ p, int N, int W,
+                                            const TilePlace<D, BY, MR>& tp, unsigned base,
+                                            const float (&v)[RY][2]) {
+  const __amdgpu_buffer_rsrc_t rs = plane_rsrc(p, N);
+#pragma unroll
+  for (int j = 0; j < RY; ++j) {
+    if (!((tp.own_r >> j) & 1u)) continue;
+    const unsigned o = base + unsigned(j * W) * 4u;
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v[j][0]), rs, o, 0, 0);
+    if (INSIDE || tp.own_c[1])
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v[j][1]), rs, o + 4u, 0, 0);
+  }
 }
 
 // One tile of one fused outer iteration (the body of k_fused and of the
@@ -331,6 +337,8 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
 #pragma unroll
         for (int k = 0; k < 2; ++k) q[j][k] *= a.lam;
     }
+  }
+  if constexpr (MODE != TV_ONLY) {
 #pragma unroll
     for (int j = 0; j < RY; ++j) yb[j][0] = yb[j][1] = 0.f;
   }
@@ -416,11 +424,11 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
                                                true);
 
     if (owns_any) {
+      float bv[RY][2];
 #pragma unroll
       for (int j = 0; j < RY; ++j) {
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-          const unsigned so = owned_offset(tp, j, k, W);
           float xn = out[j][k];
           if constexpr (MODE == ADMM) {
             const float2 im = img[j * 64];
@@ -434,7 +442,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
                 err = fmaf(e, e, err);
               }
             }
-            if (a.write_b) st_at(bt, N, so, xn - v);
+            bv[j][k] = xn - v;
             yb[j][k] = fmaf(ph[j][k], 2.f * xn - v, yb[j][k]);
           } else if constexpr (MODE != TV_ONLY) {
             if constexpr (IQA) {
@@ -445,7 +453,15 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
             }
             yb[j][k] = fmaf(ph[j][k], xn, yb[j][k]);
           }
-          st_at(xo, N, so, xn);
+          out[j][k] = xn;
+        }
+      }
+      if (tp.own_c[0]) {
+        unsigned base = unsigned(off0) * 4u;
+        compute_here(base);
+        store_owned<D, BY, MR, INSIDE>(xo, N, W, tp, base, out);
+        if constexpr (MODE == ADMM) {
+          if (a.write_b) store_owned<D, BY, MR, INSIDE>(bt, N, W, tp, base, bv);
         }
       }
     }

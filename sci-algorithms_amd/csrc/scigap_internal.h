@@ -90,6 +90,7 @@ __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcp
 // compiler may sink a running sum into a later wave-uniform branch and keep
 // every term of the sum live until then.
 __device__ __forceinline__ void compute_here(float& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void compute_here(unsigned& v) { asm volatile("" : "+v"(v)); }
 
 // Sum over the 64 lanes of a wavefront, result valid in every lane: DPP
 // within each 16-lane row (pairs, quads, half-row and row mirrors), then two
