@@ -24,8 +24,9 @@
 // The TV output out_D = v + G^T p_D starts the chain: vbar = xbar, pbar_D = G xbar.
 //
 // Activations are checkpointed once per outer iteration (x_k and y1'); the
-// inner TV states are recomputed in the backward pass (forward updates with
-// every p_u kept for that one outer iteration).  Layout (T,H,W) per sample.
+// TV step of one outer iteration is reversed by one fused tiled kernel
+// (tv_bwd.hip) that recomputes the inner dual fields in registers.  Layout
+// (T,H,W) per sample.
 
 #include <cmath>
 
@@ -35,30 +36,8 @@ namespace scigap {
 
 namespace {
 
-constexpr int BX = 64, BYR = 4;
+constexpr int BX = 64, BYR = 4;  // 256-thread blocks of the per-pixel kernels
 constexpr float kTau = 0.25f;
-
-struct Plane3 {
-  int H, W, T;
-};
-
-__device__ __forceinline__ int2 pix() {
-  return make_int2(blockIdx.x * BX + int(threadIdx.x % BX), blockIdx.y * BYR + int(threadIdx.x / BX));
-}
-
-// G^T q at (r,c) for a dual field q (two planes q0, q1 of one frame).
-__device__ __forceinline__ float gt(const float* q0, const float* q1, int r, int c, int H, int W) {
-  const size_t o = size_t(r) * W + c;
-  float d = 0.f;
-  if (q0) {
-    const float up = r > 0 ? q0[o - W] : 0.f;
-    const float me = r < H - 1 ? q0[o] : 0.f;
-    const float lf = c > 0 ? q1[o - 1] : 0.f;
-    const float m1 = c < W - 1 ? q1[o] : 0.f;
-    d = (up - me) + (lf - m1);
-  }
-  return d;
-}
 
 // Block reduction of one float into a double accumulator (scaled).
 __device__ __forceinline__ void block_atomic(float v, double* dst, double scale) {
@@ -89,94 +68,6 @@ __global__ void __launch_bounds__(256) k_ub_prep(const float* __restrict__ x,
     const float lr = lam * r;
     for (int t = 0; t < T; ++t) v[t * N + i] = fmaf(lr, phi[t * N + i], x[t * N + i]);
   }
-}
-
-// One forward Chambolle update keeping p_{u+1} (p_in null for u = 0).
-__global__ void __launch_bounds__(256) k_ub_tvf(const float* __restrict__ v,
-                                                const float* __restrict__ p_in,
-                                                float* __restrict__ p_out, float tw, Plane3 g) {
-  const int2 q = pix();
-  const int r = q.y, c = q.x, t = blockIdx.z;
-  if (r >= g.H || c >= g.W) return;
-  const size_t N = size_t(g.H) * g.W, o = size_t(r) * g.W + c;
-  const float* vt = v + t * N;
-  const float* a0 = p_in ? p_in + size_t(t) * 2 * N : nullptr;
-  const float* a1 = a0 ? a0 + N : nullptr;
-  const float out = vt[o] + gt(a0, a1, r, c, g.H, g.W);
-  const float g0 = r < g.H - 1 ? (vt[o + g.W] + gt(a0, a1, r + 1, c, g.H, g.W)) - out : 0.f;
-  const float g1 = c < g.W - 1 ? (vt[o + 1] + gt(a0, a1, r, c + 1, g.H, g.W)) - out : 0.f;
-  const float n = fast_sqrt(fmaf(g0, g0, g1 * g1));
-  const float inv = fast_rcp(fmaf(tw, n, 1.f));
-  float* b0 = p_out + size_t(t) * 2 * N;
-  b0[o] = fmaf(-kTau, g0, a0 ? a0[o] : 0.f) * inv;
-  b0[N + o] = fmaf(-kTau, g1, a1 ? a1[o] : 0.f) * inv;
-}
-
-// Reverse of update u, part A: from pbar_{u+1} = abar_{u+1} + G obar_{u+1}
-// (abar_next null for the last update) compute abar_u and gbar_u, and add
-// the weight adjoint (dc/dw = -tau/w^2) to *wbar.
-__global__ void __launch_bounds__(256) k_ub_tvb_a(const float* __restrict__ v,
-                                                  const float* __restrict__ p_u,
-                                                  const float* __restrict__ abar_next,
-                                                  const float* __restrict__ obar_next,
-                                                  float* __restrict__ abar,
-                                                  float* __restrict__ gbar, float tw, float w,
-                                                  double* wbar, Plane3 g) {
-  const int2 q = pix();
-  const int r = q.y, c = q.x, t = blockIdx.z;
-  const bool in = r < g.H && c < g.W;
-  float cb = 0.f;
-  if (in) {
-    const size_t N = size_t(g.H) * g.W, o = size_t(r) * g.W + c;
-    const float* vt = v + t * N;
-    const float* a0 = p_u ? p_u + size_t(t) * 2 * N : nullptr;
-    const float* a1 = a0 ? a0 + N : nullptr;
-    const float out = vt[o] + gt(a0, a1, r, c, g.H, g.W);
-    const float g0 = r < g.H - 1 ? (vt[o + g.W] + gt(a0, a1, r + 1, c, g.H, g.W)) - out : 0.f;
-    const float g1 = c < g.W - 1 ? (vt[o + 1] + gt(a0, a1, r, c + 1, g.H, g.W)) - out : 0.f;
-    const float n = fast_sqrt(fmaf(g0, g0, g1 * g1));
-    const float ct = tw;  // c = tau / w
-    const float s = fast_rcp(fmaf(ct, n, 1.f));
-    const float pa0 = fmaf(-kTau, g0, a0 ? a0[o] : 0.f);
-    const float pa1 = fmaf(-kTau, g1, a1 ? a1[o] : 0.f);
-    // pbar_{u+1} = abar_{u+1} + G obar_{u+1}
-    const float* ob = obar_next + t * N;
-    float pb0 = r < g.H - 1 ? ob[o + g.W] - ob[o] : 0.f;
-    float pb1 = c < g.W - 1 ? ob[o + 1] - ob[o] : 0.f;
-    if (abar_next) {
-      const float* an = abar_next + size_t(t) * 2 * N;
-      pb0 += an[o];
-      pb1 += an[N + o];
-    }
-    const float ab0 = pb0 * s, ab1 = pb1 * s;
-    const float sbar = fmaf(pb0, pa0, pb1 * pa1);
-    const float nbar = -ct * s * s * sbar;
-    cb = -n * s * s * sbar;
-    const float k = n > 0.f ? nbar / n : 0.f;
-    float* ao = abar + size_t(t) * 2 * N;
-    ao[o] = ab0;
-    ao[N + o] = ab1;
-    float* go = gbar + size_t(t) * 2 * N;
-    go[o] = fmaf(k, g0, -kTau * ab0);
-    go[N + o] = fmaf(k, g1, -kTau * ab1);
-  }
-  block_atomic(cb, wbar, -double(kTau) / (double(w) * double(w)));
-}
-
-// Reverse of update u, part B: obar_u = G^T gbar_u; vbar += obar_u
-// (vbar = xbar + obar on the first reverse step, when `first`).
-__global__ void __launch_bounds__(256) k_ub_tvb_b(const float* __restrict__ gbar,
-                                                  float* __restrict__ obar,
-                                                  const float* __restrict__ xbar,
-                                                  float* __restrict__ vbar, int first, Plane3 g) {
-  const int2 q = pix();
-  const int r = q.y, c = q.x, t = blockIdx.z;
-  if (r >= g.H || c >= g.W) return;
-  const size_t N = size_t(g.H) * g.W, o = size_t(r) * g.W + c;
-  const float* g0 = gbar + size_t(t) * 2 * N;
-  const float ov = gt(g0, g0 + N, r, c, g.H, g.W);
-  obar[t * N + o] = ov;
-  vbar[t * N + o] = (first ? xbar[t * N + o] : vbar[t * N + o]) + ov;
 }
 
 // Reverse of the projection v = x + lambda*r*Phi, r = (y1' - yb)/S,
@@ -276,12 +167,15 @@ __global__ void k_measure_bwd(const float* __restrict__ ybar, const float* __res
 
 }  // namespace
 
+int tv_backward_tiles(int D, const float* v, const float* xbar, float* vbar, int64_t H, int64_t W,
+                      int64_t T, float w, double* wbar, hipStream_t s);
+
 size_t unrolled_backward_workspace(int64_t H, int64_t W, int64_t T, int D) {
+  (void)D;
   const size_t N = size_t(H) * W, V = N * size_t(T);
-  // v, r, p_1..p_D, abar x2, gbar, obar x2, vbar, xbar x2, y1bar x2
-  return align256(V * 4) + align256(N * 4) + size_t(D) * align256(2 * V * 4) +
-         2 * align256(2 * V * 4) + align256(2 * V * 4) + 2 * align256(V * 4) + align256(V * 4) +
-         2 * align256(V * 4) + 2 * align256(N * 4);
+  // v, r, vbar, xbar x2, y1bar x2
+  return align256(V * 4) + align256(N * 4) + align256(V * 4) + 2 * align256(V * 4) +
+         2 * align256(N * 4);
 }
 
 // Backward through K unrolled iterations of one sample.
@@ -301,16 +195,9 @@ int unrolled_backward(int64_t H, int64_t W, int64_t T, int K, int D, const float
   };
   float* v = take(V * 4);
   float* r = take(N * 4);
-  float* P[9] = {nullptr};
-  for (int u = 1; u <= D; ++u) P[u] = take(2 * V * 4);
-  float* abar[2] = {take(2 * V * 4), take(2 * V * 4)};
-  float* gbar = take(2 * V * 4);
-  float* obar[2] = {take(V * 4), take(V * 4)};
   float* vbar = take(V * 4);
   float* xbar[2] = {take(V * 4), take(V * 4)};
   float* y1bar[2] = {take(N * 4), take(N * 4)};
-  const Plane3 g{int(H), int(W), int(T)};
-  const dim3 sgrid(unsigned((W + BX - 1) / BX), unsigned((H + BYR - 1) / BYR), unsigned(T));
 
   k_fill_f32<<<grid1(N), 256, 0, s>>>(grad_y, N, 0.f);
   SCIGAP_LAUNCH_CHECK("grad fill");
@@ -318,28 +205,18 @@ int unrolled_backward(int64_t H, int64_t W, int64_t T, int K, int D, const float
   const float* y1b_in = nullptr;
   int xsel = 0, ysel = 0;
   for (int k = K - 1; k >= 0; --k) {
-    const float lam = lambdas[k], w = weights[k], tw = float(0.25 / double(w));
+    const float lam = lambdas[k], w = weights[k];
     const float* xk = ckpt_x + size_t(k) * xstride;
     const float* y1p = ckpt_y1 + size_t(k) * y1stride;  // y1' of iteration k
     {
       ProfScope ps("unrolled_bwd_recompute", s);
       k_ub_prep<<<grid1(N), 256, 0, s>>>(xk, y1p, phi, S, lam, v, r, N, int(T));
-      for (int u = 0; u < D; ++u)
-        k_ub_tvf<<<sgrid, BX * BYR, 0, s>>>(v, u == 0 ? nullptr : P[u], P[u + 1], tw, g);
     }
     SCIGAP_LAUNCH_CHECK("unrolled recompute");
     {
       ProfScope ps("unrolled_bwd_tv", s);
-      for (int u = D - 1; u >= 0; --u) {
-        const int cur = u & 1, nxt = (u + 1) & 1;
-        const float* an = u == D - 1 ? nullptr : abar[nxt];
-        const float* on = u == D - 1 ? xb_in : obar[nxt];
-        k_ub_tvb_a<<<sgrid, BX * BYR, 0, s>>>(v, u == 0 ? nullptr : P[u], an, on, abar[cur], gbar,
-                                               tw, w, grad_weight + k, g);
-        k_ub_tvb_b<<<sgrid, BX * BYR, 0, s>>>(gbar, obar[cur], xb_in, vbar, u == D - 1 ? 1 : 0, g);
-      }
+      if (int rc = tv_backward_tiles(D, v, xb_in, vbar, H, W, T, w, grad_weight + k, s)) return rc;
     }
-    SCIGAP_LAUNCH_CHECK("unrolled tv backward");
     {
       ProfScope ps("unrolled_bwd_proj", s);
       k_ub_proj<<<grid1(N), 256, 0, s>>>(vbar, xk, phi, S, r, lam, y1b_in, xbar[xsel],
