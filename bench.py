@@ -250,6 +250,60 @@ def batched(args):
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
 
+    # The same loop with the TV in PyTorch operations as well: a direct GPU
+    # port of the reference (denoise_tv_chambolle of scikit-image 0.18.3,
+    # _denoise.py:341-393, per frame with its energy stop).
+    def tv_torch(img, weight, eps, n_iter_max):
+        T_ = img.shape[0]
+        p = torch.zeros((2,) + img.shape, device=img.device)
+        g = torch.zeros_like(p)
+        d = torch.zeros_like(img)
+        result = torch.empty_like(img)
+        done = torch.zeros(T_, dtype=torch.bool, device=img.device)
+        e_init = e_prev = None
+        size = img.shape[1] * img.shape[2]
+        for i in range(n_iter_max):
+            if i > 0:
+                d = -p[0] - p[1]
+                d[:, 1:] += p[0, :, :-1]
+                d[:, :, 1:] += p[1, :, :, :-1]
+                out = img + d
+            else:
+                out = img
+            e = (d * d).sum((1, 2))
+            g[0, :, :-1] = out[:, 1:] - out[:, :-1]
+            g[1, :, :, :-1] = out[:, :, 1:] - out[:, :, :-1]
+            norm = torch.sqrt((g * g).sum(0))
+            e = (e + weight * norm.sum((1, 2))) / size
+            p_new = (p - 0.25 * g) / (1.0 + (0.25 / weight) * norm)
+            if i == 0:
+                e_init = e_prev = e
+            else:
+                stop = (e_prev - e).abs() < eps * e_init
+                newly = stop & ~done
+                result = torch.where(newly[:, None, None], out, result)
+                done = done | stop
+                e_prev = e
+            p = torch.where(done[None, :, None, None], p, p_new)
+        return torch.where(done[:, None, None], result, out)
+
+    def plain_torch(y, phi, ps):
+        x = y.unsqueeze(0) * phi
+        y1 = torch.zeros_like(y)
+        for _ in range(K):
+            yb = (x * phi).sum(0)
+            y1 = y1 + (y - yb)
+            x = x + phi * ((y1 - yb) / ps).unsqueeze(0)
+            x = tv_torch(x, tvw, 2e-4, 5)
+        return x
+
+    plain_torch(*dev[0][:3])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xs_torch = [plain_torch(y, phi, ps) for (y, phi, ps, _) in dev]
+    torch.cuda.synchronize()
+    t_torch = time.perf_counter() - t0
+
     plan = BatchPlan(dev, iters=K, tv_weight=tvw, use_graph=True)
     plan.run()
     torch.cuda.synchronize()
@@ -270,6 +324,7 @@ def batched(args):
     ps_seq = mean_psnr([s.x for s in singles])
     ps_batch = mean_psnr(plan.x)
     ps_plain = mean_psnr(xs_plain)
+    ps_torch = mean_psnr(xs_torch)
     plan.close()
     sizes = {}
     for (size, _p) in draws:
@@ -289,7 +344,11 @@ def batched(args):
         "plain_gpu": {"value": pfi / t_plain, "ms": t_plain * 1e3, "mean_psnr_db": ps_plain,
                       "mode": "reference loop as separate device operations per solve: "
                               "torch A/At/residual, one library TV call per iteration"},
-        "speedup": t_seq / t_batch, "speedup_vs_plain_gpu": t_plain / t_batch, "unit": UNIT,
+        "plain_torch": {"value": pfi / t_torch, "ms": t_torch * 1e3, "mean_psnr_db": ps_torch,
+                        "mode": "reference loop in PyTorch operations per solve, TV included "
+                                "(Chambolle with the per-frame energy stop)"},
+        "speedup": t_seq / t_batch, "speedup_vs_plain_gpu": t_plain / t_batch,
+        "speedup_vs_plain_torch": t_torch / t_batch, "unit": UNIT,
     }
 
 
@@ -330,12 +389,22 @@ def unrolled(args):
         fwd, bwd = fwd / reps, bwd / reps
         total = fwd + bwd
         peak = torch.cuda.max_memory_allocated() - base
+        # Library kernels of one more forward + backward, by kind (HIP events).
+        from pnp_sci import _lib
+
+        def step():
+            xx = unrolled_gap_tv(m, tt, lt, wt, logits=logits)
+            ((xx - tt) ** 2).mean().backward()
+            torch.cuda.synchronize()
+        prof = _lib.kernel_profile(step)
+        kernels = {k: {"launches": n, "ms": ms} for k, (n, ms) in sorted(prof.items())}
         B, T, H, W = truth.shape
         out[name] = {"fwd_bwd_ms": total * 1e3, "fwd_ms": fwd * 1e3, "bwd_ms": bwd * 1e3,
                      "peak_extra_GiB": peak / 2 ** 30, "batch": B, "shape": [T, H, W],
                      "iters": K, "tv_iter_max": 5,
                      "mask": "sigmoid(logits), learnable" if logits else "shifting binary, fixed",
-                     "pixel_frame_iter_per_s": B * T * H * W * K / total}
+                     "pixel_frame_iter_per_s": B * T * H * W * K / total,
+                     "kernels_one_fwd_bwd": kernels}
         del x, tt, m, lt, wt
         torch.cuda.empty_cache()
     return out

@@ -34,7 +34,8 @@ struct FusedArgs {
   float* b;             // ADMM b (T,H,W)
   double* energy;       // [T][D] energies of this iteration
   float* yb;            // (H,W) sum_t Phi_t*x_t^{new} on owned pixels (for the repair)
-  int* stop;            // [T] exit iteration per frame (written by k_decide)
+  int* stop;            // [T] exit iteration per frame (written by k_decide);
+                        // null: no energy test (eps = 0), no decision/repair
   int* nexit;           // frames of this iteration that stopped early (k_decide)
   int* stats;           // optional copy of *nexit for the caller
   int H, W, T;

@@ -206,6 +206,9 @@ int launch_fused(const FusedArgs& a0, hipStream_t s, int tile_rows, int engine) 
     rc = launch_mode<D, BY>(a0, s, engine);
   }
   if (rc) return rc;
+  // No stop array: the caller runs without the energy test (eps = 0), so no
+  // frame can stop early and there is nothing to decide or repair.
+  if (!a0.stop) return SCIGAP_OK;
   FusedArgs a = a0;
   const unsigned grid = set_tiles<D, BY, MR>(a);
   {
@@ -328,7 +331,7 @@ int unrolled_forward_iterations(int64_t H, int64_t W, int64_t T, int K, int D,
     a.y1_in = y1s + size_t(k) * y1stride;
     a.y1_out = k + 1 < K ? y1s + size_t(k + 1) * y1stride : nullptr;
     a.energy = energy + size_t(k) * T * D;
-    a.stop = stop;
+    a.stop = nullptr;  // eps = 0: no early exit, no decision or repair pass
     a.nexit = nexit + k;
     a.yb_part = yb_part;
     a.H = int(H);

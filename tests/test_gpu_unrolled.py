@@ -76,6 +76,8 @@ CASES = [
     (2, 130, 260, 6, 3, 3, True),    # several tiles, depth 2
     (1, 48, 64, 5, 2, 7, False),     # fixed mask, depth 6
     (3, 40, 40, 2, 5, 2, False),     # depth 1
+    (1, 150, 270, 3, 2, 6, True),    # depth 5, interior and border backward tiles
+    (2, 100, 200, 3, 2, 4, False),   # depth 3
 ]
 
 
