@@ -99,7 +99,10 @@ __global__ void __launch_bounds__(64 * BY) k_fix_batch(const FusedArgs* __restri
     const int2 it = items[e];
     const FusedArgs& a = descs[it.x];
     if (*a.nexit == 0) continue;
-    fused_fix_tile<D, BY, MR>(a, it.y >= 0 ? it.y : ~it.y, sh);
+    if (it.y >= 0)
+      fused_fix_tile<D, BY, MR, true>(a, it.y, sh);
+    else
+      fused_fix_tile<D, BY, MR, false>(a, ~it.y, sh);
     __syncthreads();
   }
 }

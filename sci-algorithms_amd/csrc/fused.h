@@ -585,7 +585,7 @@ struct FixLds {
   double red64[BY];
 };
 
-template <int D, int BY, int MR>
+template <int D, int BY, int MR, bool INSIDE>
 __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, FixLds<D, BY, MR>& sh) {
   using G = TileGeom<D, BY, MR>;
   constexpr int RY = G::RY;
@@ -599,7 +599,7 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
   const LanePos& lp = tp.lp;
   const int off0 = tp.off0;
   EdgeMasks em;
-  em.init<RY>(lp, H, W);
+  if constexpr (!INSIDE) em.init<RY>(lp, H, W);
   const bool tv_only = a.mode == TV_ONLY;
 
   float q[RY][2];
@@ -607,11 +607,11 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
 #pragma unroll
     for (int j = 0; j < RY; ++j) q[j][0] = q[j][1] = 0.f;
   } else {
-    load_plane<RY, false>(a.r_in, q, lp, off0, H, W);
+    load_plane<RY, INSIDE>(a.r_in, q, lp, off0, H, W);
   }
   if (a.from_y) {
     float yv[RY][2];
-    load_plane<RY, false>(a.y, yv, lp, off0, H, W);
+    load_plane<RY, INSIDE>(a.y, yv, lp, off0, H, W);
 #pragma unroll
     for (int j = 0; j < RY; ++j)
 #pragma unroll
@@ -633,25 +633,23 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
     const float* phit = a.phi + size_t(t) * N;
     const float* xt = a.x_in + size_t(t) * N;
     float* xo = a.x_out + size_t(t) * N;
-    float ph[RY][2], img[RY][2], xv[RY][2] = {};
+    float ph[RY][2], xv[RY][2] = {};
     if (tv_only) {
 #pragma unroll
       for (int j = 0; j < RY; ++j) ph[j][0] = ph[j][1] = 0.f;
     } else {
-      load_plane<RY, false>(phit, ph, lp, off0, H, W);
+      load_plane<RY, INSIDE>(phit, ph, lp, off0, H, W);
     }
-    if (!a.from_y) load_plane<RY, false>(xt, xv, lp, off0, H, W);
+    if (!a.from_y) load_plane<RY, INSIDE>(xt, xv, lp, off0, H, W);
     float2* const simg = s_img + lp.w * RY * 64 + lp.lane;
 #pragma unroll
-    for (int j = 0; j < RY; ++j) {
-#pragma unroll
-      for (int k = 0; k < 2; ++k)
-        img[j][k] = fmaf(q[j][k], ph[j][k], xv[j][k]);  // xv = 0 when from_y
-      simg[j * 64] = make_float2(img[j][0], img[j][1]);
-    }
+    for (int j = 0; j < RY; ++j)  // xv = 0 when from_y
+      simg[j * 64] = make_float2(fmaf(q[j][0], ph[j][0], xv[j][0]),
+                                 fmaf(q[j][1], ph[j][1], xv[j][1]));
     float out[RY][2];
     float acc[NH * 8];
-    tv_run_from_zero<D, RY, BY, NH, true>(simg, out, nup, lds, lp, em, a.weight, a.tw, acc, false);
+    tv_run_from_zero<D, RY, BY, NH, !INSIDE>(simg, out, nup, lds, lp, em, a.weight, a.tw, acc,
+                                             false);
     const float* trt = a.truth ? a.truth + size_t(t) * N : nullptr;
     float* bt = a.b ? a.b + size_t(t) * N : nullptr;
 #pragma unroll
@@ -665,7 +663,10 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
         if (!tv_only) {
           if (a.mode == ADMM) {
             if (a.clip) xn = fminf(fmaxf(xn, 0.f), 1.f);
-            if (a.write_b) bt[o] = xn - img[j][k];
+            if (a.write_b) {
+              const float2 im = simg[j * 64];  // this lane's own slot
+              bt[o] = xn - (k ? im.y : im.x);
+            }
             dyb[j][k] = fmaf(2.f * ph[j][k], xn - xold, dyb[j][k]);
           } else {
             if (trt) {  // replace the main pass's error of this pixel
@@ -697,13 +698,20 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
   if (a.truth && a.err && a.mode != ADMM && !tv_only) commit_err<BY>(err, a.err, red64, lp.w, lp.lane);
 }
 
+// For D <= 4 at least 4 waves per SIMD: two 8-wave workgroups per CU
+// (128 VGPRs); deeper tiles need more registers than that.
 template <int D, int BY, int MR>
-__global__ void __launch_bounds__(64 * BY) k_fused_fix(FusedArgs a) {
+__global__ void __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(D <= 4 ? 4 : 1)))
+k_fused_fix(FusedArgs a) {
   __shared__ FixLds<D, BY, MR> sh;
   if (*a.nexit == 0) return;  // the common case: one scalar load per workgroup
   const int tile = tile_of_block(blockIdx.x, a.per_xcd);
   if (tile >= a.ntiles) return;
-  fused_fix_tile<D, BY, MR>(a, tile, sh);
+  // Interior tiles run without masks or bounds checks (as in the main pass).
+  if (tile_inside<D, BY, MR>(tile, a.tiles_x, a.H, a.W))
+    fused_fix_tile<D, BY, MR, true>(a, tile, sh);
+  else
+    fused_fix_tile<D, BY, MR, false>(a, tile, sh);
 }
 
 // Initial residual: X_0 = x0 (or At(y)), yb = A(X_0), then

@@ -57,6 +57,13 @@ template <int D> struct FusedShape {
   static constexpr int MR = 1;
 };
 
+// Repair-kernel tiles (independent of the main pass' tiles: it recomputes
+// whole frames): for D = 4 the 8-wave shape of the default main pass, so
+// that two workgroups share a CU.
+template <int D> struct FixShape {
+  static constexpr int BY = D == 4 ? 8 : FusedShape<D>::BY;
+};
+
 // Tile grid of the (D, BY) geometry written into the launch arguments.
 template <int D, int BY, int MR>
 unsigned set_tiles(FusedArgs& a) {
@@ -210,7 +217,8 @@ int launch_fused(const FusedArgs& a0, hipStream_t s, int tile_rows, int engine) 
   // frame can stop early and there is nothing to decide or repair.
   if (!a0.stop) return SCIGAP_OK;
   FusedArgs a = a0;
-  const unsigned grid = set_tiles<D, BY, MR>(a);
+  constexpr int FBY = FixShape<D>::BY;
+  const unsigned grid = set_tiles<D, FBY, MR>(a);
   {
     ProfScope ps("fused_decide", s);
     k_decide<D><<<1, 256, 0, s>>>(a.energy, a.T, a.eps, a.stop, a.nexit, a.stats);
@@ -218,7 +226,7 @@ int launch_fused(const FusedArgs& a0, hipStream_t s, int tile_rows, int engine) 
   SCIGAP_LAUNCH_CHECK("early-exit decision launch");
   {
     ProfScope ps("fused_fix", s);
-    k_fused_fix<D, BY, MR><<<grid, 64 * BY, 0, s>>>(a);
+    k_fused_fix<D, FBY, MR><<<grid, 64 * FBY, 0, s>>>(a);
   }
   SCIGAP_LAUNCH_CHECK("fused early-exit launch");
   return SCIGAP_OK;
