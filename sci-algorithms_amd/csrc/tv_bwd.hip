@@ -14,9 +14,9 @@
 //     obar = G^T gbar,  vbar += obar,  pbar_u = abar + G obar.
 //
 // Nothing but v, xbar and vbar touches memory: the dual field p_u that step u
-// needs is recomputed in registers from p_0 = 0 (u forward updates, the same
-// arithmetic as the forward engine), so a step costs u + 1/2 forward updates
-// plus its own two stencils.  Layout: a workgroup of 16 waves holds a
+// needs is recomputed in registers (the same arithmetic as the forward
+// engine), from the forward state after D/2 updates, kept in registers, for
+// u >= D/2 and from p_0 = 0 otherwise.  Layout: a workgroup of 16 waves holds a
 // 128 x 64 load tile of one frame, lane l of wave w columns c0+2l, c0+2l+1 of
 // rows r0+4w .. r0+4w+3; the stencil state of a step is in registers, v and
 // the running adjoints vbar, pbar in LDS planes (each lane reads back only
@@ -132,6 +132,32 @@ __device__ __forceinline__ void gtdiv(const Rows& a0, const Rows& a1, Xchg& X, R
   }
 }
 
+// One forward update on the register tile: p <- (p - tau g)/(1 + c|g|),
+// g = G out; then out = v + G^T p (v from this lane's LDS slots).
+template <bool EDGE>
+__device__ __forceinline__ void fwd_update(Rows& out, Rows& p0, Rows& p1, Xchg& X,
+                                           const EdgeMasks& em, const float2* vimg, float tw) {
+  constexpr int RY = kBwdRY;
+  float g0[RY][2], g1[RY][2];
+  gdiff<EDGE>(out, X, em, g0, g1);
+#pragma unroll
+  for (int j = 0; j < RY; ++j)
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const float n = fast_sqrt(fmaf(g0[j][k], g0[j][k], g1[j][k] * g1[j][k]));
+      const float s = fast_rcp(fmaf(tw, n, 1.f));
+      p0[j][k] = fmaf(-kTauB, g0[j][k], p0[j][k]) * s;
+      p1[j][k] = fmaf(-kTauB, g1[j][k], p1[j][k]) * s;
+    }
+  gtdiv(p0, p1, X, out);
+#pragma unroll
+  for (int j = 0; j < RY; ++j) {
+    const float2 im = vimg[j * 64];
+    out[j][0] = im.x + out[j][0];
+    out[j][1] = im.y + out[j][1];
+  }
+}
+
 template <int D, bool INSIDE>
 __device__ __forceinline__ void bwd_tile(const BwdArgs& a, int tile, int t, BwdLds& L) {
   using G = BwdGeom<D>;
@@ -184,40 +210,40 @@ __device__ __forceinline__ void bwd_tile(const BwdArgs& a, int tile, int t, BwdL
   float cb = 0.f;  // sum of -n s^2 sbar over owned pixels
   const float tw = a.tw;
 
+  // Forward state after M = D/2 updates (p_M, out_M), kept in registers:
+  // steps u >= M resume from it instead of from p_0 = 0.
+  constexpr int M = D / 2;
+  float pm0[RY][2], pm1[RY][2], outm[RY][2];
+#pragma unroll
+  for (int j = 0; j < RY; ++j) {
+    const float2 im = vimg[j * 64];
+    outm[j][0] = im.x;
+    outm[j][1] = im.y;
+    pm0[j][0] = pm0[j][1] = pm1[j][0] = pm1[j][1] = 0.f;
+  }
+#pragma unroll 1
+  for (int i = 0; i < M; ++i) fwd_update<!INSIDE>(outm, pm0, pm1, X, em, vimg, tw);
+
   // The step and recompute loops stay rolled: unrolled, the compiler
   // interleaves steps and runs out of registers.
 #pragma unroll 1
   for (int u = D - 1; u >= 0; --u) {
-    // Recompute p_u and out_u (u forward updates from p = 0, out = v).
+    // Recompute p_u and out_u: u - M updates from the checkpoint, or u
+    // updates from p = 0, out = v.
+    const bool resume = u >= M;
     float p0[RY][2], p1[RY][2], out[RY][2];
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
       const float2 im = vimg[j * 64];
-      out[j][0] = im.x;
-      out[j][1] = im.y;
-      p0[j][0] = p0[j][1] = p1[j][0] = p1[j][1] = 0.f;
-    }
-#pragma unroll 1
-    for (int i = 0; i < u; ++i) {
-      float g0[RY][2], g1[RY][2];
-      gdiff<!INSIDE>(out, X, em, g0, g1);
 #pragma unroll
-      for (int j = 0; j < RY; ++j)
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          const float n = fast_sqrt(fmaf(g0[j][k], g0[j][k], g1[j][k] * g1[j][k]));
-          const float s = fast_rcp(fmaf(tw, n, 1.f));
-          p0[j][k] = fmaf(-kTauB, g0[j][k], p0[j][k]) * s;
-          p1[j][k] = fmaf(-kTauB, g1[j][k], p1[j][k]) * s;
-        }
-      gtdiv(p0, p1, X, out);
-#pragma unroll
-      for (int j = 0; j < RY; ++j) {
-        const float2 im = vimg[j * 64];
-        out[j][0] = im.x + out[j][0];
-        out[j][1] = im.y + out[j][1];
+      for (int k = 0; k < 2; ++k) {
+        out[j][k] = resume ? outm[j][k] : (k ? im.y : im.x);
+        p0[j][k] = resume ? pm0[j][k] : 0.f;
+        p1[j][k] = resume ? pm1[j][k] : 0.f;
       }
     }
+#pragma unroll 1
+    for (int i = resume ? M : 0; i < u; ++i) fwd_update<!INSIDE>(out, p0, p1, X, em, vimg, tw);
     // Step u of the reverse pass.
     float gb0[RY][2], gb1[RY][2];
     {
@@ -237,10 +263,11 @@ __device__ __forceinline__ void bwd_tile(const BwdArgs& a, int tile, int t, BwdL
           ab0[k] = pb0 * s;
           ab1[k] = pb1 * s;
           const float sbar = fmaf(pb0, a0, pb1 * a1);
-          const float nbar = -tw * s * s * sbar;
+          const float q = s * s * sbar;
           const bool own = own_c[k] && ((own_r >> j) & 1u);
-          cb += own ? -n * s * s * sbar : 0.f;
-          const float kk = n > 0.f ? nbar / n : 0.f;
+          cb -= own ? n * q : 0.f;
+          // nbar / n with nbar = -c q (hardware reciprocal, about 1 ulp)
+          const float kk = n > 0.f ? -tw * q * fast_rcp(n) : 0.f;
           gb0[j][k] = fmaf(kk, g0[j][k], -kTauB * ab0[k]);
           gb1[j][k] = fmaf(kk, g1[j][k], -kTauB * ab1[k]);
         }
