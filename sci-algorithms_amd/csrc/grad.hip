@@ -28,7 +28,11 @@
 // (tv_bwd.hip) that recomputes the inner dual fields in registers.  Layout
 // (T,H,W) per sample.
 
+#include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
 
 #include "scigap_internal.h"
 
@@ -130,6 +134,13 @@ __global__ void k_fill_f32(float* p, int64_t n, float v) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x)
     p[i] = v;
+}
+
+// dst += src (elementwise).
+__global__ void k_add_f32(float* __restrict__ dst, const float* __restrict__ src, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    dst[i] += src[i];
 }
 
 unsigned grid1(int64_t n) {
@@ -272,21 +283,33 @@ CkptLayout ckpt_layout(const scigap_unrolled& u) {
   return c;
 }
 
-struct UWorkspace {
-  float* phi = nullptr;     // sigmoid(logits), when logits
-  float* S = nullptr;       // phi_sum
+// Samples of a batch run concurrently on up to kLanes streams ("lanes");
+// every lane has its own scratch, and lanes 1.. their own dL/dphi partial.
+constexpr int kLanes = 4;
+
+struct ULane {
   float* r2 = nullptr;      // two residual planes
   double* energy = nullptr; // K*T*D
   int* stop = nullptr;      // T
   int* nexit = nullptr;     // K
   float* yb_part = nullptr; // [min(T,8)][H*W] partial yb of frame-split launches
-  float* phibar = nullptr;  // (T,H,W) dL/dphi, when logits
   float* ybar = nullptr;    // (H,W) dL/dy of one sample
+  float* phibar = nullptr;  // (T,H,W) dL/dphi partial (lanes >= 1)
+  void* bwd = nullptr;
+};
+
+struct UWorkspace {
+  float* phi = nullptr;     // sigmoid(logits), when logits
+  float* S = nullptr;       // phi_sum
+  float* phibar = nullptr;  // (T,H,W) dL/dphi, when logits
   double* gl = nullptr;     // K, when the caller passes no grad_lambda
   double* gw = nullptr;     // K, when the caller passes no grad_weight
-  void* bwd = nullptr;
+  int lanes = 1;
+  ULane lane[kLanes];
   size_t bytes = 0;
 };
+
+int lanes_of(const scigap_unrolled& u) { return u.batch < kLanes ? u.batch : kLanes; }
 
 UWorkspace carve(const scigap_unrolled& u, char* base) {
   const size_t N = size_t(u.H) * u.W, V = N * size_t(u.T), K = size_t(u.iters);
@@ -300,19 +323,94 @@ UWorkspace carve(const scigap_unrolled& u, char* base) {
   };
   if (u.logits) w.phi = reinterpret_cast<float*>(take(V * 4));
   w.S = reinterpret_cast<float*>(take(N * 4));
-  w.r2 = reinterpret_cast<float*>(take(2 * N * 4));
-  w.energy = reinterpret_cast<double*>(take(K * size_t(u.T) * size_t(D) * 8));
-  w.stop = reinterpret_cast<int*>(take(size_t(u.T) * 4));
-  w.nexit = reinterpret_cast<int*>(take(K * 4));
-  w.yb_part = reinterpret_cast<float*>(take(size_t(u.T < 8 ? u.T : 8) * N * 4));
   if (u.logits) w.phibar = reinterpret_cast<float*>(take(V * 4));
-  w.ybar = reinterpret_cast<float*>(take(N * 4));
   w.gl = reinterpret_cast<double*>(take(K * 8));
   w.gw = reinterpret_cast<double*>(take(K * 8));
-  w.bwd = take(unrolled_backward_workspace(u.H, u.W, u.T, D));
+  w.lanes = lanes_of(u);
+  for (int l = 0; l < w.lanes; ++l) {
+    ULane& L = w.lane[l];
+    L.r2 = reinterpret_cast<float*>(take(2 * N * 4));
+    L.energy = reinterpret_cast<double*>(take(K * size_t(u.T) * size_t(D) * 8));
+    L.stop = reinterpret_cast<int*>(take(size_t(u.T) * 4));
+    L.nexit = reinterpret_cast<int*>(take(K * 4));
+    L.yb_part = reinterpret_cast<float*>(take(size_t(u.T < 8 ? u.T : 8) * N * 4));
+    L.ybar = reinterpret_cast<float*>(take(N * 4));
+    if (l > 0) L.phibar = reinterpret_cast<float*>(take(V * 4));
+    L.bwd = take(unrolled_backward_workspace(u.H, u.W, u.T, D));
+  }
   w.bytes = off;
   return w;
 }
+
+// Non-blocking, default-priority streams for the lanes, created once per
+// device and kept for the life of the process.
+struct LanePool {
+  std::mutex mu;
+  std::map<int, std::vector<hipStream_t>> streams;
+};
+LanePool& lane_pool() {
+  static LanePool p;
+  return p;
+}
+
+// Fork: lanes 1.. of a call wait for everything enqueued on `s` so far.
+// Join: `s` waits for every lane.  With one lane, or while `s` is being
+// captured into a graph, all samples stay on `s` (no fork).
+class Lanes {
+ public:
+  Lanes(int want, hipStream_t s) : s_(s) {
+    st_[0] = s;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (want <= 1 || hipStreamIsCapturing(s, &cs) != hipSuccess ||
+        cs != hipStreamCaptureStatusNone)
+      return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    LanePool& p = lane_pool();
+    std::lock_guard<std::mutex> lock(p.mu);
+    std::vector<hipStream_t>& v = p.streams[dev];
+    while (int(v.size()) < kLanes - 1) {
+      hipStream_t t = nullptr;
+      if (hipStreamCreateWithFlags(&t, hipStreamNonBlocking) != hipSuccess) break;
+      v.push_back(t);
+    }
+    const int n = std::min<int>(want, 1 + int(v.size()));
+    if (n <= 1 || hipEventCreateWithFlags(&fork_, hipEventDisableTiming) != hipSuccess) return;
+    if (hipEventRecord(fork_, s) != hipSuccess) return;
+    for (int l = 1; l < n; ++l) {
+      if (hipStreamWaitEvent(v[l - 1], fork_, 0) != hipSuccess) return;
+      st_[l] = v[l - 1];
+      n_ = l + 1;
+    }
+  }
+  int count() const { return n_; }
+  hipStream_t stream(int l) const { return st_[l % n_]; }
+  int lane(int b) const { return b % n_; }
+  // Makes `s` wait for every lane; returns a status.
+  int join() {
+    for (int l = 1; l < n_; ++l) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+        return fail(SCIGAP_EHIP, "hipEventCreate failed");
+      const bool ok = hipEventRecord(e, st_[l]) == hipSuccess &&
+                      hipStreamWaitEvent(s_, e, 0) == hipSuccess;
+      (void)hipEventDestroy(e);
+      if (!ok) return fail(SCIGAP_EHIP, "lane join failed");
+    }
+    n_ = 1;
+    return SCIGAP_OK;
+  }
+  ~Lanes() {
+    join();
+    if (fork_) (void)hipEventDestroy(fork_);
+  }
+
+ private:
+  hipStream_t s_;
+  hipStream_t st_[kLanes] = {};
+  hipEvent_t fork_ = nullptr;
+  int n_ = 1;
+};
 
 int check_unrolled(const scigap_unrolled* u) {
   if (!u) return fail(SCIGAP_ENULL, "unrolled descriptor is NULL");
@@ -400,26 +498,29 @@ int scigap_unrolled_forward(const scigap_unrolled* u, const float* lambdas, cons
   const UWorkspace w = carve(*u, static_cast<char*>(workspace));
   const float* phi = nullptr;
   if (int rc = prepare_mask(*u, mask, w, &phi, s)) return rc;
+  Lanes lanes(w.lanes, s);  // sample b runs on lane b % lanes
   for (int b = 0; b < u->batch; ++b) {
+    const ULane& L = w.lane[lanes.lane(b)];
+    hipStream_t ls = lanes.stream(b);
     char* cb = static_cast<char*>(ckpt) + cl.per_sample * b;
     float* xs = reinterpret_cast<float*>(cb + cl.x);
     float* y1s = reinterpret_cast<float*>(cb + cl.y1);
     float* yb = reinterpret_cast<float*>(cb + cl.y);
     if (u->measure) {
-      if (int rc = scigap_forward(meas + size_t(b) * V, phi, yb, u->H, u->W, u->T, s)) return rc;
+      if (int rc = scigap_forward(meas + size_t(b) * V, phi, yb, u->H, u->W, u->T, ls)) return rc;
     } else {
       hipError_t e = hipMemcpyAsync(yb, meas + size_t(b) * N, size_t(N) * 4,
-                                    hipMemcpyDeviceToDevice, s);
+                                    hipMemcpyDeviceToDevice, ls);
       if (e != hipSuccess) return hip_fail(e, "copy y");
     }
-    if (int rc = scigap_adjoint(yb, phi, xs, u->H, u->W, u->T, s)) return rc;  // x_0 = At(y)
+    if (int rc = scigap_adjoint(yb, phi, xs, u->H, u->W, u->T, ls)) return rc;  // x_0 = At(y)
     if (int rc = unrolled_forward_iterations(u->H, u->W, u->T, u->iters, u->tv_iter_max - 1,
                                              lambdas, weights, yb, phi, w.S, xs, cl.xstride,
-                                             y1s, cl.y1stride, x_out + size_t(b) * V, w.r2,
-                                             w.energy, w.stop, w.nexit, w.yb_part, s))
+                                             y1s, cl.y1stride, x_out + size_t(b) * V, L.r2,
+                                             L.energy, L.stop, L.nexit, L.yb_part, ls))
       return rc;
   }
-  return SCIGAP_OK;
+  return lanes.join();
 }
 
 int scigap_unrolled_backward(const scigap_unrolled* u, const float* lambdas, const float* weights,
@@ -450,28 +551,45 @@ int scigap_unrolled_backward(const scigap_unrolled* u, const float* lambdas, con
   if (int rc = fill_f64(gl, K, 0.0, s)) return rc;
   if (int rc = fill_f64(gw, K, 0.0, s)) return rc;
   float* phibar = grad_mask ? (u->logits ? w.phibar : grad_mask) : nullptr;
+  Lanes lanes(w.lanes, s);
+  // dL/dphi: lane 0 accumulates into phibar, lanes 1.. into their own
+  // partials, summed into phibar after the join.
+  float* lane_phibar[kLanes] = {phibar};
+  for (int l = 1; l < lanes.count(); ++l) lane_phibar[l] = phibar ? w.lane[l].phibar : nullptr;
   if (phibar) {
-    k_fill_f32<<<grid1(V), 256, 0, s>>>(phibar, V, 0.f);
-    SCIGAP_LAUNCH_CHECK("gradient fill");
+    for (int l = 0; l < lanes.count(); ++l) {
+      k_fill_f32<<<grid1(V), 256, 0, lanes.stream(l)>>>(lane_phibar[l], V, 0.f);
+      SCIGAP_LAUNCH_CHECK("gradient fill");
+    }
   }
   for (int b = 0; b < u->batch; ++b) {
+    const int l = lanes.lane(b);
+    const ULane& L = w.lane[l];
+    hipStream_t ls = lanes.stream(b);
+    float* pb = lane_phibar[l];
     const char* cb = static_cast<const char*>(ckpt) + cl.per_sample * b;
     const float* xs = reinterpret_cast<const float*>(cb + cl.x);
     const float* y1s = reinterpret_cast<const float*>(cb + cl.y1);
     const float* yb = reinterpret_cast<const float*>(cb + cl.y);
     if (int rc = unrolled_backward(u->H, u->W, u->T, K, D, lambdas, weights, yb, phi, w.S, xs,
-                                   cl.xstride, y1s, cl.y1stride, grad_x + size_t(b) * V, phibar, w.ybar, gl, gw, w.bwd,
-                                   s))
+                                   cl.xstride, y1s, cl.y1stride, grad_x + size_t(b) * V, pb,
+                                   L.ybar, gl, gw, L.bwd, ls))
       return rc;
-    if (u->measure && phibar) {
-      k_measure_bwd<<<grid1(N), 256, 0, s>>>(w.ybar, meas + size_t(b) * V, phibar, N, int(u->T));
+    if (u->measure && pb) {
+      k_measure_bwd<<<grid1(N), 256, 0, ls>>>(L.ybar, meas + size_t(b) * V, pb, N, int(u->T));
       SCIGAP_LAUNCH_CHECK("measurement backward launch");
     }
     if (!u->measure && grad_meas) {
-      hipError_t e = hipMemcpyAsync(grad_meas + size_t(b) * N, w.ybar, size_t(N) * 4,
-                                    hipMemcpyDeviceToDevice, s);
+      hipError_t e = hipMemcpyAsync(grad_meas + size_t(b) * N, L.ybar, size_t(N) * 4,
+                                    hipMemcpyDeviceToDevice, ls);
       if (e != hipSuccess) return hip_fail(e, "copy dL/dy");
     }
+  }
+  const int nl = lanes.count();
+  if (int rc = lanes.join()) return rc;
+  for (int l = 1; l < nl && phibar; ++l) {
+    k_add_f32<<<grid1(V), 256, 0, s>>>(phibar, lane_phibar[l], V);
+    SCIGAP_LAUNCH_CHECK("gradient lane sum");
   }
   if (grad_mask && u->logits) {
     k_sigmoid_bwd<<<grid1(V), 256, 0, s>>>(w.phibar, phi, grad_mask, V);
