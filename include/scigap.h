@@ -264,11 +264,13 @@ typedef struct scigap_unrolled {
 } scigap_unrolled;
 
 /* Bytes of the checkpoint buffer one forward call fills and the matching
- * backward call reads: per sample, x_k (T,H,W) and y1_k (H,W) at the start
+ * backward call reads: per sample, x_k (T,H,W) and the residual r_k (H,W)
  * of every outer iteration, and y. */
 size_t scigap_unrolled_checkpoint_bytes(const scigap_unrolled* u);
 
-/* Workspace of a forward or a backward call (the same size for both). */
+/* Workspace of a forward or a backward call (the same size for both);
+ * with batch > 1 it holds the scratch of up to 4 concurrently running
+ * samples. */
 size_t scigap_unrolled_workspace_bytes(const scigap_unrolled* u);
 
 /* Forward: x_out (batch,T,H,W) = x_K of every sample.  lambdas and weights

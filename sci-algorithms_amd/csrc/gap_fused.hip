@@ -309,22 +309,20 @@ int check_params(const scigap_params* p) {
 // per-iteration lambda_k / weight_k and no early exit (eps = 0 never meets
 // |E_prev - E| < eps*E_0).  xs holds x_0 on entry; iteration k reads x_k from
 // xs + k*xstride and writes x_{k+1} to xs + (k+1)*xstride (the last one to
-// x_out), and y1' of iteration k lands in y1s + k*y1stride (strides in
-// floats).  r2 holds two residual planes, energy K*T*D doubles, stop T ints,
-// nexit K ints.
+// x_out); the residual r_k it uses is written to rs + k*rstride (strides in
+// floats).  y1 ping-pongs between the two planes of y1b; energy holds K*T*D
+// doubles, nexit K ints.
 int unrolled_forward_iterations(int64_t H, int64_t W, int64_t T, int K, int D,
                                 const float* lambdas, const float* weights, const float* y,
                                 const float* phi, const float* S, float* xs, size_t xstride,
-                                float* y1s, size_t y1stride, float* x_out, float* r2,
-                                double* energy, int* stop, int* nexit, float* yb_part,
-                                hipStream_t s) {
+                                float* y1b, float* x_out, float* rs, size_t rstride,
+                                double* energy, int* nexit, float* yb_part, hipStream_t s) {
   const int64_t N = H * W;
   if (D < 1 || D > kMaxFused) return fail(SCIGAP_EUNSUPPORTED, "unrolled depth %d", D);
   if (int rc = fill_f64(energy, K * int(T) * D, 0.0, s)) return rc;
-  float* r[2] = {r2, r2 + N};
   {
     ProfScope ps("init", s);
-    k_init<<<grid_pixels(N), 256, 0, s>>>(xs, phi, y, S, r[0], y1s, N, int(T), GAP_ACC, 0.f);
+    k_init<<<grid_pixels(N), 256, 0, s>>>(xs, phi, y, S, rs, y1b, N, int(T), GAP_ACC, 0.f);
   }
   SCIGAP_LAUNCH_CHECK("unrolled init launch");
   for (int k = 0; k < K; ++k) {
@@ -334,10 +332,10 @@ int unrolled_forward_iterations(int64_t H, int64_t W, int64_t T, int K, int D,
     a.phi = phi;
     a.y = y;
     a.phi_sum = S;
-    a.r_in = r[k & 1];
-    a.r_out = r[(k + 1) & 1];
-    a.y1_in = y1s + size_t(k) * y1stride;
-    a.y1_out = k + 1 < K ? y1s + size_t(k + 1) * y1stride : nullptr;
+    a.r_in = rs + size_t(k) * rstride;
+    a.r_out = k + 1 < K ? rs + size_t(k + 1) * rstride : nullptr;
+    a.y1_in = y1b + size_t(k & 1) * N;
+    a.y1_out = k + 1 < K ? y1b + size_t((k + 1) & 1) * N : nullptr;
     a.energy = energy + size_t(k) * T * D;
     a.stop = nullptr;  // eps = 0: no early exit, no decision or repair pass
     a.nexit = nexit + k;

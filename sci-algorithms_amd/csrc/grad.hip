@@ -23,10 +23,11 @@
 //   obar_u = G^T gbar_u,  vbar += obar_u,  pbar_u = abar_u + G obar_u.
 // The TV output out_D = v + G^T p_D starts the chain: vbar = xbar, pbar_D = G xbar.
 //
-// Activations are checkpointed once per outer iteration (x_k and y1'); the
-// TV step of one outer iteration is reversed by one fused tiled kernel
-// (tv_bwd.hip) that recomputes the inner dual fields in registers.  Layout
-// (T,H,W) per sample.
+// Activations are checkpointed once per outer iteration (x_k, y1' and the
+// residual r_k the forward used); the TV step of one outer iteration is
+// reversed by one fused tiled kernel (tv_bwd.hip) that forms v from x_k, r_k
+// and Phi as the forward did and recomputes the inner dual fields in
+// registers.  Layout (T,H,W) per sample.
 
 #include <algorithm>
 #include <cmath>
@@ -53,24 +54,6 @@ __device__ __forceinline__ void block_atomic(float v, double* dst, double scale)
     double tot = 0.0;
     for (int w = 0; w < BX * BYR / 64; ++w) tot += part[w];
     atomicAdd(dst, tot * scale);
-  }
-}
-
-// v = x_k + lambda * r * Phi with r = (y1' - yb)/S, yb = sum_t Phi_t x_k.
-__global__ void __launch_bounds__(256) k_ub_prep(const float* __restrict__ x,
-                                                 const float* __restrict__ y1p,
-                                                 const float* __restrict__ phi,
-                                                 const float* __restrict__ S, float lam,
-                                                 float* __restrict__ v, float* __restrict__ r_out,
-                                                 int64_t N, int T) {
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
-       i += int64_t(gridDim.x) * blockDim.x) {
-    float yb = 0.f;
-    for (int t = 0; t < T; ++t) yb = fmaf(phi[t * N + i], x[t * N + i], yb);
-    const float r = (y1p[i] - yb) / S[i];
-    r_out[i] = r;
-    const float lr = lam * r;
-    for (int t = 0; t < T; ++t) v[t * N + i] = fmaf(lr, phi[t * N + i], x[t * N + i]);
   }
 }
 
@@ -178,22 +161,22 @@ __global__ void k_measure_bwd(const float* __restrict__ ybar, const float* __res
 
 }  // namespace
 
-int tv_backward_tiles(int D, const float* v, const float* xbar, float* vbar, int64_t H, int64_t W,
-                      int64_t T, float w, double* wbar, hipStream_t s);
+int tv_backward_tiles(int D, const float* x, const float* phi, const float* r, float lam,
+                      const float* xbar, float* vbar, int64_t H, int64_t W, int64_t T, float w,
+                      double* wbar, hipStream_t s);
 
 size_t unrolled_backward_workspace(int64_t H, int64_t W, int64_t T, int D) {
   (void)D;
   const size_t N = size_t(H) * W, V = N * size_t(T);
-  // v, r, vbar, xbar x2, y1bar x2
-  return align256(V * 4) + align256(N * 4) + align256(V * 4) + 2 * align256(V * 4) +
-         2 * align256(N * 4);
+  // vbar, xbar x2, y1bar x2
+  return align256(V * 4) + 2 * align256(V * 4) + 2 * align256(N * 4);
 }
 
 // Backward through K unrolled iterations of one sample.
 int unrolled_backward(int64_t H, int64_t W, int64_t T, int K, int D, const float* lambdas,
                       const float* weights, const float* y, const float* phi, const float* S,
-                      const float* ckpt_x, size_t xstride, const float* ckpt_y1,
-                      size_t y1stride, const float* grad_x,
+                      const float* ckpt_x, size_t xstride, const float* ckpt_r, size_t rstride,
+                      const float* grad_x,
                       float* grad_phi, float* grad_y, double* grad_lambda, double* grad_weight,
                       void* ws, hipStream_t s) {
   const int64_t N = H * W, V = N * T;
@@ -204,8 +187,6 @@ int unrolled_backward(int64_t H, int64_t W, int64_t T, int K, int D, const float
     off += align256(bytes);
     return reinterpret_cast<float*>(q);
   };
-  float* v = take(V * 4);
-  float* r = take(N * 4);
   float* vbar = take(V * 4);
   float* xbar[2] = {take(V * 4), take(V * 4)};
   float* y1bar[2] = {take(N * 4), take(N * 4)};
@@ -218,15 +199,12 @@ int unrolled_backward(int64_t H, int64_t W, int64_t T, int K, int D, const float
   for (int k = K - 1; k >= 0; --k) {
     const float lam = lambdas[k], w = weights[k];
     const float* xk = ckpt_x + size_t(k) * xstride;
-    const float* y1p = ckpt_y1 + size_t(k) * y1stride;  // y1' of iteration k
-    {
-      ProfScope ps("unrolled_bwd_recompute", s);
-      k_ub_prep<<<grid1(N), 256, 0, s>>>(xk, y1p, phi, S, lam, v, r, N, int(T));
-    }
-    SCIGAP_LAUNCH_CHECK("unrolled recompute");
+    const float* r = ckpt_r + size_t(k) * rstride;  // residual the forward used
     {
       ProfScope ps("unrolled_bwd_tv", s);
-      if (int rc = tv_backward_tiles(D, v, xb_in, vbar, H, W, T, w, grad_weight + k, s)) return rc;
+      if (int rc = tv_backward_tiles(D, xk, phi, r, lam, xb_in, vbar, H, W, T, w, grad_weight + k,
+                                     s))
+        return rc;
     }
     {
       ProfScope ps("unrolled_bwd_proj", s);
@@ -258,26 +236,25 @@ int fill_f64(double* p, int n, double v, hipStream_t s);
 int unrolled_forward_iterations(int64_t H, int64_t W, int64_t T, int K, int D,
                                 const float* lambdas, const float* weights, const float* y,
                                 const float* phi, const float* S, float* xs, size_t xstride,
-                                float* y1s, size_t y1stride, float* x_out, float* r2,
-                                double* energy, int* stop, int* nexit, float* yb_part,
-                                hipStream_t s);
+                                float* y1b, float* x_out, float* rs, size_t rstride,
+                                double* energy, int* nexit, float* yb_part, hipStream_t s);
 
 namespace {
 
-// Checkpoint of one sample: x_0..x_{K-1}, y1'_0..y1'_{K-1}, y; every plane
-// set starts on a 256-byte boundary (strides in floats).
+// Checkpoint of one sample: x_0..x_{K-1}, r_0..r_{K-1}, y; every plane set
+// starts on a 256-byte boundary (strides in floats).
 struct CkptLayout {
-  size_t x = 0, y1 = 0, y = 0, per_sample = 0, total = 0, xstride = 0, y1stride = 0;
+  size_t x = 0, r = 0, y = 0, per_sample = 0, total = 0, xstride = 0, rstride = 0;
 };
 
 CkptLayout ckpt_layout(const scigap_unrolled& u) {
   const size_t N = size_t(u.H) * u.W, V = N * size_t(u.T), K = size_t(u.iters);
   CkptLayout c;
   c.xstride = align256(V * 4) / 4;
-  c.y1stride = align256(N * 4) / 4;
+  c.rstride = align256(N * 4) / 4;
   c.x = 0;
-  c.y1 = K * c.xstride * 4;
-  c.y = c.y1 + K * c.y1stride * 4;
+  c.r = K * c.xstride * 4;
+  c.y = c.r + K * c.rstride * 4;
   c.per_sample = c.y + align256(N * 4);
   c.total = c.per_sample * size_t(u.batch);
   return c;
@@ -288,9 +265,8 @@ CkptLayout ckpt_layout(const scigap_unrolled& u) {
 constexpr int kLanes = 4;
 
 struct ULane {
-  float* r2 = nullptr;      // two residual planes
+  float* y1b = nullptr;     // two y1 planes (forward)
   double* energy = nullptr; // K*T*D
-  int* stop = nullptr;      // T
   int* nexit = nullptr;     // K
   float* yb_part = nullptr; // [min(T,8)][H*W] partial yb of frame-split launches
   float* ybar = nullptr;    // (H,W) dL/dy of one sample
@@ -329,9 +305,8 @@ UWorkspace carve(const scigap_unrolled& u, char* base) {
   w.lanes = lanes_of(u);
   for (int l = 0; l < w.lanes; ++l) {
     ULane& L = w.lane[l];
-    L.r2 = reinterpret_cast<float*>(take(2 * N * 4));
+    L.y1b = reinterpret_cast<float*>(take(2 * N * 4));
     L.energy = reinterpret_cast<double*>(take(K * size_t(u.T) * size_t(D) * 8));
-    L.stop = reinterpret_cast<int*>(take(size_t(u.T) * 4));
     L.nexit = reinterpret_cast<int*>(take(K * 4));
     L.yb_part = reinterpret_cast<float*>(take(size_t(u.T < 8 ? u.T : 8) * N * 4));
     L.ybar = reinterpret_cast<float*>(take(N * 4));
@@ -504,7 +479,7 @@ int scigap_unrolled_forward(const scigap_unrolled* u, const float* lambdas, cons
     hipStream_t ls = lanes.stream(b);
     char* cb = static_cast<char*>(ckpt) + cl.per_sample * b;
     float* xs = reinterpret_cast<float*>(cb + cl.x);
-    float* y1s = reinterpret_cast<float*>(cb + cl.y1);
+    float* rs = reinterpret_cast<float*>(cb + cl.r);
     float* yb = reinterpret_cast<float*>(cb + cl.y);
     if (u->measure) {
       if (int rc = scigap_forward(meas + size_t(b) * V, phi, yb, u->H, u->W, u->T, ls)) return rc;
@@ -516,8 +491,8 @@ int scigap_unrolled_forward(const scigap_unrolled* u, const float* lambdas, cons
     if (int rc = scigap_adjoint(yb, phi, xs, u->H, u->W, u->T, ls)) return rc;  // x_0 = At(y)
     if (int rc = unrolled_forward_iterations(u->H, u->W, u->T, u->iters, u->tv_iter_max - 1,
                                              lambdas, weights, yb, phi, w.S, xs, cl.xstride,
-                                             y1s, cl.y1stride, x_out + size_t(b) * V, L.r2,
-                                             L.energy, L.stop, L.nexit, L.yb_part, ls))
+                                             L.y1b, x_out + size_t(b) * V, rs, cl.rstride,
+                                             L.energy, L.nexit, L.yb_part, ls))
       return rc;
   }
   return lanes.join();
@@ -569,10 +544,10 @@ int scigap_unrolled_backward(const scigap_unrolled* u, const float* lambdas, con
     float* pb = lane_phibar[l];
     const char* cb = static_cast<const char*>(ckpt) + cl.per_sample * b;
     const float* xs = reinterpret_cast<const float*>(cb + cl.x);
-    const float* y1s = reinterpret_cast<const float*>(cb + cl.y1);
+    const float* rs = reinterpret_cast<const float*>(cb + cl.r);
     const float* yb = reinterpret_cast<const float*>(cb + cl.y);
     if (int rc = unrolled_backward(u->H, u->W, u->T, K, D, lambdas, weights, yb, phi, w.S, xs,
-                                   cl.xstride, y1s, cl.y1stride, grad_x + size_t(b) * V, pb,
+                                   cl.xstride, rs, cl.rstride, grad_x + size_t(b) * V, pb,
                                    L.ybar, gl, gw, L.bwd, ls))
       return rc;
     if (u->measure && pb) {

@@ -13,7 +13,7 @@
 //     gbar = -tau abar + (nbar / n_u) g_u   (0 where n_u = 0),
 //     obar = G^T gbar,  vbar += obar,  pbar_u = abar + G obar.
 //
-// Nothing but v, xbar and vbar touches memory: the dual field p_u that step u
+// Nothing but x, Phi, r, xbar and vbar touches memory: the dual field p_u that step u
 // needs is recomputed in registers (the same arithmetic as the forward
 // engine), from the forward state after D/2 updates, kept in registers, for
 // u >= D/2 and from p_0 = 0 otherwise.  Layout: a workgroup of 16 waves holds a
@@ -50,7 +50,10 @@ struct BwdGeom {
 };
 
 struct BwdArgs {
-  const float* v;     // (T,H,W) TV input
+  const float* x;     // (T,H,W) x_k of the outer iteration
+  const float* phi;   // (T,H,W) mask
+  const float* r;     // (H,W) residual r_k the forward used
+  float lam;          // lambda_k
   const float* xbar;  // (T,H,W) adjoint of the TV output
   float* vbar;        // (T,H,W) adjoint of the TV input
   double* wbar;       // adjoint of the TV weight (accumulated)
@@ -186,8 +189,18 @@ __device__ __forceinline__ void bwd_tile(const BwdArgs& a, int tile, int t, BwdL
   EdgeMasks em;
   if constexpr (!INSIDE) em.init<RY>(lp, H, W);
 
+  // TV input v = x_k + (lambda_k r_k) Phi, formed as the forward pass forms it.
   float vr[RY][2], xb[RY][2];
-  load_plane<RY, INSIDE>(a.v + size_t(t) * N, vr, lp, off0, H, W);
+  {
+    float xv[RY][2], ph[RY][2];
+    load_plane<RY, INSIDE>(a.r, vr, lp, off0, H, W);
+    load_plane<RY, INSIDE>(a.x + size_t(t) * N, xv, lp, off0, H, W);
+    load_plane<RY, INSIDE>(a.phi + size_t(t) * N, ph, lp, off0, H, W);
+#pragma unroll
+    for (int j = 0; j < RY; ++j)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) vr[j][k] = fmaf(vr[j][k] * a.lam, ph[j][k], xv[j][k]);
+  }
   load_plane<RY, INSIDE>(a.xbar + size_t(t) * N, xb, lp, off0, H, W);
   // This lane's own slots of the LDS planes (row j at [j * 64]).
   const int slot = lp.w * RY * 64 + lp.lane;
@@ -334,11 +347,16 @@ int launch_tv_bwd(BwdArgs a, int T, hipStream_t s) {
 }  // namespace
 
 // vbar = (d out_D / d v)^T xbar for every frame, and wbar += xbar . d out_D / dw,
-// for D fixed Chambolle updates with weight w (D in [1, 6]; H*W < 2^30 - 8).
-int tv_backward_tiles(int D, const float* v, const float* xbar, float* vbar, int64_t H, int64_t W,
-                      int64_t T, float w, double* wbar, hipStream_t s) {
+// for D fixed Chambolle updates with weight w on v = x + (lam r) Phi (D in
+// [1, 6]; H*W < 2^30 - 8).
+int tv_backward_tiles(int D, const float* x, const float* phi, const float* r, float lam,
+                      const float* xbar, float* vbar, int64_t H, int64_t W, int64_t T, float w,
+                      double* wbar, hipStream_t s) {
   BwdArgs a{};
-  a.v = v;
+  a.x = x;
+  a.phi = phi;
+  a.r = r;
+  a.lam = lam;
   a.xbar = xbar;
   a.vbar = vbar;
   a.wbar = wbar;

@@ -85,9 +85,9 @@ def test_unrolled_sizes_and_validation(dll):
     u = _lib.Unrolled(H=64, W=48, T=4, batch=2, iters=3, tv_iter_max=5, logits=1, measure=1)
     N, V = 64 * 48, 64 * 48 * 4
     ck = dll.scigap_unrolled_checkpoint_bytes(ctypes.byref(u))
-    assert ck >= 2 * (3 * V * 4 + 3 * N * 4 + N * 4)      # x_k, y1_k, y per sample
+    assert ck >= 2 * (3 * V * 4 + 3 * N * 4 + N * 4)      # x_k, r_k, y per sample
     ws = dll.scigap_unrolled_workspace_bytes(ctypes.byref(u))
-    assert ws >= 4 * 2 * V * 4                            # the recomputed dual fields p_1..p_4
+    assert ws >= 2 * 3 * V * 4                            # two lanes: vbar, xbar x2 each
     for field, bad in (("tv_iter_max", 8), ("tv_iter_max", 1), ("iters", 0), ("batch", 0)):
         v = _lib.Unrolled.from_buffer_copy(u)
         setattr(v, field, bad)
