@@ -18,7 +18,10 @@
  *    different streams with disjoint buffers may run concurrently.
  *  - Every entry point returns 0 on success or a negative SCIGAP_E* code;
  *    scigap_last_error() then returns a message (per host thread).  No C++
- *    exception crosses this boundary.
+ *    exception crosses this boundary.  Scalar arguments are validated on the
+ *    host (non-finite values are SCIGAP_EINVAL); device arrays are not
+ *    scanned, and non-finite values in them propagate as in the reference's
+ *    NumPy arithmetic.
  *  - Scratch memory is a caller-provided device workspace whose size the
  *    matching *_workspace_bytes() function returns; the library allocates
  *    no device memory on these paths.

@@ -31,6 +31,7 @@
 // (x, r and y1 are double-buffered) and corrects yb and the epilogue.
 
 #include <algorithm>
+#include <cmath>
 
 #include "fused.h"
 
@@ -296,7 +297,13 @@ int check_params(const scigap_params* p) {
     return fail(SCIGAP_EINVAL, "unknown method %d", p->method);
   if (p->iters < 0) return fail(SCIGAP_EINVAL, "iters must be >= 0");
   if (p->tv_iter_max < 1) return fail(SCIGAP_EINVAL, "tv_iter_max must be >= 1");
-  if (!(p->tv_weight > 0.f)) return fail(SCIGAP_EINVAL, "tv_weight must be > 0");
+  if (!(p->tv_weight > 0.f) || !std::isfinite(p->tv_weight))
+    return fail(SCIGAP_EINVAL, "tv_weight must be finite and > 0");
+  // Scalars are checked here, on the host; non-finite values inside the
+  // device arrays propagate through the arithmetic as in the reference.
+  if (!std::isfinite(p->lambda_) || !std::isfinite(p->gamma) || !std::isfinite(p->tv_eps) ||
+      !std::isfinite(p->tv_weight_decay) || !std::isfinite(p->gamma_decay))
+    return fail(SCIGAP_EINVAL, "lambda_, gamma, tv_eps and the decay factors must be finite");
   if (p->tile_rows != 0 && p->tile_rows != 8 && p->tile_rows != 12 && p->tile_rows != 16)
     return fail(SCIGAP_EINVAL, "tile_rows must be 0, 8, 12 or 16");
   if (p->engine < 0 || p->engine > 6) return fail(SCIGAP_EINVAL, "engine must be in [0, 6]");
@@ -497,7 +504,9 @@ int scigap_tv_chambolle_depth(const float* image, float* out, int64_t H, int64_t
   if (int rc = check_shape(H, W, T)) return rc;
   if (!image || !out) return fail(SCIGAP_ENULL, "scigap_tv_chambolle: null image/out");
   if (n_iter_max < 1) return fail(SCIGAP_EINVAL, "n_iter_max must be >= 1");
-  if (!(weight > 0.f)) return fail(SCIGAP_EINVAL, "weight must be > 0");
+  if (!(weight > 0.f) || !std::isfinite(weight))
+    return fail(SCIGAP_EINVAL, "weight must be finite and > 0");
+  if (!std::isfinite(eps)) return fail(SCIGAP_EINVAL, "eps must be finite");
   if (depth < 0 || depth > kMaxFused) return fail(SCIGAP_EINVAL, "depth must be in [0, 6]");
   if (H * W > (int64_t(1) << 30) - 8)
     return fail(SCIGAP_EINVAL, "H*W must be <= 2^30 - 8 (buffer-addressed planes)");

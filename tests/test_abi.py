@@ -79,6 +79,16 @@ def test_workspace_sizes_and_params(dll):
                           ctypes.c_void_p(1 << 23), None, ctypes.c_void_p(1 << 24), None, None,
                           None, None, 0, None)
     assert rc == -4 and b"workspace" in dll.scigap_last_error()
+    # non-finite scalars are rejected on the host
+    for field in ("lambda_", "gamma", "tv_weight", "tv_eps", "tv_weight_decay"):
+        q = _lib.Params.from_buffer_copy(p)
+        setattr(q, field, float("nan") if field != "tv_weight" else float("inf"))
+        assert dll.scigap_solve_workspace_bytes(q) == 0, field
+        assert b"finite" in dll.scigap_last_error(), field
+    for w, e in ((float("inf"), 2e-4), (0.1, float("nan"))):
+        assert dll.scigap_tv_chambolle(ctypes.c_void_p(256), ctypes.c_void_p(4096), 4, 4, 1, w, e,
+                                       5, None, None, 0, None) == -1
+        assert b"finite" in dll.scigap_last_error()
 
 
 def test_unrolled_sizes_and_validation(dll):
