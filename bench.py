@@ -509,6 +509,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--no-unrolled", action="store_true", help="skip the unrolled model")
+    ap.add_argument("--only-unrolled", action="store_true",
+                    help="run only the unrolled-model leg and print its JSON (for profiling)")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
 
@@ -518,6 +520,9 @@ def main():
         sys.exit(2)
     if args.traffic_child:
         traffic_child()
+        return
+    if args.only_unrolled:
+        print(json.dumps({"unrolled": unrolled(args)}), flush=True)
         return
 
     from pnp_sci.synthetic import make_problem
