@@ -114,7 +114,7 @@ def compare_ssim(X, Y, data_range=1.0):
     X = np.asarray(X, np.float64)
     Y = np.asarray(Y, np.float64)
     if min(X.shape) < 7:
-        raise ValueError("win_size exceeds image extent")
+        raise ValueError("win_size exceeds image extent.  If the input is a multichannel (color) image, set multichannel=True.")
     win, npix = 7, 49.0
     cov = npix / (npix - 1.0)
     ux, uy = uniform_filter(X, win), uniform_filter(Y, win)
@@ -257,8 +257,9 @@ def _final_metrics(x, X_orig):
     if X_orig is None:
         return [], []
     psnr_ = [compare_psnr(X_orig[..., m], x[..., m]) for m in range(x.shape[-1])]
-    ssim_ = ([compare_ssim(X_orig[..., m], x[..., m]) for m in range(x.shape[-1])]
-             if min(x.shape[:2]) >= 7 else [])
+    # the reference calls compare_ssim for every frame (so frames smaller
+    # than the 7x7 window raise scikit-image's ValueError there)
+    ssim_ = [compare_ssim(X_orig[..., m], x[..., m]) for m in range(x.shape[-1])]
     return psnr_, ssim_
 
 

@@ -26,9 +26,12 @@ def psnr_frames(truth, x, data_range: float = 1.0):
 
 
 def ssim_frames(truth, x, data_range: float = 1.0):
-    """Mean SSIM per frame of two (T,H,W) device tensors (scigap_ssim)."""
+    """Mean SSIM per frame of two (T,H,W) device tensors (scigap_ssim).
+    Frames smaller than the 7x7 window raise scikit-image's ValueError."""
     import torch
     T, H, W = x.shape
+    if min(H, W) < 7:
+        raise ValueError("win_size exceeds image extent.  If the input is a multichannel (color) image, set multichannel=True.")
     out = torch.empty(T, dtype=torch.float64, device=x.device)
     L = _lib.lib()
     _lib.check(L.scigap_ssim(truth.data_ptr(), x.data_ptr(), out.data_ptr(), H, W, T,

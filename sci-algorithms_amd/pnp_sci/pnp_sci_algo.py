@@ -210,7 +210,7 @@ def _run(method, y, Phi_sum, A, At, *, _lambda, accelerate, gamma, clip, denoise
     psnr_, ssim_ = [], []
     if X_orig is not None:
         psnr_ = psnr_frames(truth, x)
-        ssim_ = ssim_frames(truth, x) if min(H, W) >= 7 else []
+        ssim_ = ssim_frames(truth, x)  # raises for frames under 7x7, as the reference does
     return to_host_like(thw_to_hwt(x), y), psnr_, ssim_, psnr_all
 
 
@@ -295,7 +295,7 @@ def gap_denoise_bayer(y_bayer, Phi_bayer, _lambda=1, accelerate=True, denoiser='
         xt = hwt_to_thw(x_bayer)
         tt = hwt_to_thw(truth)
         psnr_ = psnr_frames(tt, xt)
-        ssim_ = ssim_frames(tt, xt) if min(nrow, ncol) >= 7 else []
+        ssim_ = ssim_frames(tt, xt)
     return to_host_like(x_bayer, Phi_bayer), psnr_, ssim_, psnr_all
 
 

@@ -78,6 +78,7 @@ CASES = [
     (3, 40, 40, 2, 5, 2, False),     # depth 1
     (1, 150, 270, 3, 2, 6, True),    # depth 5, interior and border backward tiles
     (2, 100, 200, 3, 2, 4, False),   # depth 3
+    (2, 5, 9, 2, 2, 5, True),        # frames smaller than the reverse tile's halo
 ]
 
 
