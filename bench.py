@@ -356,6 +356,42 @@ def batched(args):
 # Unrolled differentiable GAP-TV
 # ---------------------------------------------------------------------------
 
+def other_solves(prob):
+    """ADMM-TV on the headline problem (gamma 0.01, the reference default)
+    and the small cfg1 solve (256x256x8, Bernoulli(0.5), K=40, w=0.1): time
+    per solve with data in HBM, and the fused kernel's algorithmic bandwidth."""
+    import numpy as np
+    import torch
+    from pnp_sci import _lib
+    from pnp_sci.device import to_device
+    from pnp_sci.synthetic import config1
+    out = {}
+    cases = (("admm_2048x2048x20_K60", prob, 60, 0.05, _lib.METHOD_ADMM, (5 * 20 + 2) * 4),
+             ("gap_256x256x8_K40", config1(0), 40, 0.1, _lib.METHOD_GAP, gap_bytes_per_pixel(8)))
+    for name, pr, K, tvw, method, bpp in cases:
+        solve = Solver(to_device(pr.y), to_device(pr.mask_thw), to_device(pr.phi_sum), K, tvw)
+        solve.p.method = method
+        solve.nbytes = solve.L.scigap_solve_workspace_bytes(solve.p)
+        solve.ws = torch.empty(solve.nbytes, dtype=torch.uint8, device="cuda")
+        solve()
+        torch.cuda.synchronize()
+        reps = 3 if pr.y.size > 1 << 20 else 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            solve()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        prof = _lib.kernel_profile(solve)
+        n_it, ms_it = prof.get("fused_iteration", (1, float("nan")))
+        T, H, W = pr.mask_thw.shape
+        x = solve.x.cpu().numpy().astype(np.float64)
+        out[name] = {"ms_per_solve": dt * 1e3, "pixel_frame_iter_per_s": H * W * T * K / dt,
+                     "fused_launch_ms": ms_it / n_it,
+                     "fused_algorithmic_GBps": bpp * H * W / (ms_it / n_it * 1e-3) / 1e9,
+                     "psnr_db": 10 * math.log10(1.0 / float(np.mean((x - pr.truth_thw) ** 2)))}
+    return out
+
+
 def unrolled(args):
     import torch
     from pnp_sci.synthetic import config4, config5
@@ -544,6 +580,8 @@ def main():
         extra["batched_queue"] = batched(args)
     if not args.no_unrolled:
         extra["unrolled"] = unrolled(args)
+    if args.config == "cfg2" and not args.no_batch:
+        extra["other_solves"] = other_solves(prob)
 
     cpu, checks = cpu_leg(args, prob, solve, want_unrolled=not args.no_unrolled and
                           not args.no_check)

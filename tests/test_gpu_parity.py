@@ -229,6 +229,12 @@ def test_warm_start_and_torch_inputs(gpu):
     want = O.gap_denoise(pr.y.astype(np.float64), pr.phi_sum.astype(np.float64), A64, At64,
                          iter_max=6, x0=x0.astype(np.float64), show_iqa=False)[0]
     assert rel(got.cpu().numpy(), want) <= 1e-4
+    # staged iter_max / sigma lists (pnp_sci_algo.py:628-631) continue one state
+    staged = pnp_sci.gap_denoise(pr.y, pr.phi_sum, A, At, iter_max=[4, 2], sigma=[None, None],
+                                 x0=x0, X_orig=pr.truth)
+    flat = pnp_sci.gap_denoise(pr.y, pr.phi_sum, A, At, iter_max=6, x0=x0, X_orig=pr.truth)
+    assert np.array_equal(staged[0], flat[0]) and len(staged[3]) == 6
+    assert np.allclose(staged[3], flat[3])
 
 
 def test_forced_early_exit_inside_fused_solve(gpu):
