@@ -9,9 +9,10 @@ _denoise.py:341-393 without the energy early exit, which would make the map
 discontinuous).  Gradients flow to the mask (or its logits, phi =
 sigmoid(logits)), to lambda_k and tv_weight_k, and to the measurement y.
 
-The forward pass keeps x_k and y1 of every outer iteration (one checkpoint per
-outer iteration); the backward pass recomputes the inner TV states of one
-outer iteration at a time and runs the hand-written reverse kernels.
+The forward pass keeps x_k and the residual r_k of every outer iteration (one
+checkpoint per outer iteration); the backward pass reverses one outer
+iteration at a time with the hand-written kernels, recomputing the inner TV
+states in registers.
 
 Tensors use the library's (T,H,W) frame-major layout: mask (T,H,W), truth
 (B,T,H,W), y (B,H,W), output (B,T,H,W).
