@@ -31,6 +31,7 @@
 // (x, r and y1 are double-buffered) and corrects yb and the epilogue.
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 #include "fused.h"
@@ -102,8 +103,9 @@ constexpr int kMaxParts = 8;  // frame parts (yb_part planes in the workspace)
 // assumed to be the same model).
 template <int D, int BY, int MODE, bool IQA>
 int resident_workgroups() {
-  static int cached = 0;
-  if (cached == 0) {
+  static std::atomic<int> cached{0};  // host threads may race to fill it (same value)
+  int v = cached.load(std::memory_order_relaxed);
+  if (v == 0) {
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -111,9 +113,10 @@ int resident_workgroups() {
             &per_cu, k_fused<D, BY, 1, MODE, IQA, false, 0>, 64 * BY, 0) != hipSuccess ||
         cus <= 0 || per_cu <= 0)
       return 256;
-    cached = cus * per_cu;
+    v = cus * per_cu;
+    cached.store(v, std::memory_order_relaxed);
   }
-  return cached;
+  return v;
 }
 
 // Frame parts per tile: minimise rounds(F) * (frames per part + overhead),

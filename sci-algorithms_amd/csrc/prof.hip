@@ -5,6 +5,7 @@
 
 #include "scigap_internal.h"
 
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -20,7 +21,7 @@ struct Rec {
 };
 
 std::mutex g_mu;
-bool g_on = false;
+std::atomic<bool> g_on{false};  // read without the lock on every launch
 std::vector<Rec> g_recs;
 std::vector<hipEvent_t> g_pool;
 
