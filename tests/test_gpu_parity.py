@@ -452,3 +452,45 @@ def test_gap_denoise_bayer_matches_oracle(gpu, accelerate):
     assert np.max(np.abs(np.array(ssim_) - np.array(ws))) <= 1e-4
     with pytest.raises(ValueError):
         pnp_sci.gap_denoise_bayer(y[:-1], mask[:-1], iter_max=1)
+
+
+def test_random_shapes_engines_against_fp64_oracle(gpu):
+    """Seeded sweep over shapes (including widths around the 128-column tile
+    and odd sizes), frame counts, both methods, TV depths, tile heights and
+    engines: every solve within 1e-4 of the FP64 oracle."""
+    import torch
+    from pnp_sci import _lib
+    from pnp_sci.device import to_device
+    L = gpu
+    rng = np.random.default_rng(2024)
+    widths = [1, 2, 7, 64, 127, 128, 129, 130, 250, 255, 256, 257, 383]
+    for case in range(14):
+        H = int(rng.integers(1, 200))
+        W = int(rng.choice(widths))
+        T = int(rng.integers(1, 13))
+        method = "admm" if rng.random() < 0.35 else "gap"
+        n = int(rng.choice([2, 3, 5, 5, 5, 7]))
+        rows = int(rng.choice([0, 8, 12, 16])) if n == 5 else 0
+        engine = int(rng.integers(0, 7)) if n == 5 else 0
+        pr = problem(H, W, T, seed=case, p=float(rng.uniform(0.3, 0.7)))
+        A64, At64 = ops64(pr.mask)
+        y64, ps64 = pr.y.astype(np.float64), pr.phi_sum.astype(np.float64)
+        solver = O.gap_denoise if method == "gap" else O.admm_denoise
+        want = solver(y64, ps64, A64, At64, iter_max=5, tv_weight=0.1, tv_iter_max=n,
+                      show_iqa=False)[0]
+        p = _lib.Params()
+        L.scigap_default_params(p, H, W, T)
+        p.method = _lib.METHOD_ADMM if method == "admm" else _lib.METHOD_GAP
+        p.iters, p.tv_weight, p.tv_iter_max = 5, 0.1, n
+        p.tile_rows, p.engine = rows, engine
+        y, phi, ps = to_device(pr.y), to_device(pr.mask_thw), to_device(pr.phi_sum)
+        nb = L.scigap_solve_workspace_bytes(p)
+        ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        x = torch.empty((T, H, W), device="cuda")
+        _lib.check(L.scigap_solve(p, y.data_ptr(), phi.data_ptr(), ps.data_ptr(), None,
+                                  x.data_ptr(), None, None, None, ws.data_ptr(), nb,
+                                  _lib.stream_ptr()))
+        got = x.cpu().numpy().transpose(1, 2, 0)
+        e = rel(got, want)
+        print("case", case, (H, W, T, method, n, rows, engine), "rel %.2e" % e)
+        assert e <= 1e-4, (case, H, W, T, method, n, rows, engine, e)
