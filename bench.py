@@ -38,8 +38,8 @@ HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
     # name: (H, W, T, K, tv_weight, mask, noise_rel)
-    "cfg2": (2048, 2048, 20, 60, 0.05, "shift", 0.01),
-    "cfg1": (256, 256, 8, 40, 0.1, "bernoulli", 0.0),
+    "2048x2048x20": (2048, 2048, 20, 60, 0.05, "shift", 0.01),
+    "256x256x8": (256, 256, 8, 40, 0.1, "bernoulli", 0.0),
 }
 
 
@@ -207,11 +207,11 @@ def batched(args):
     import torch
     from pnp_sci.batch import BatchPlan
     from pnp_sci.device import to_device
-    from pnp_sci.synthetic import config3_problem, config3_sizes
+    from pnp_sci.synthetic import mixed_queue_problem, mixed_queue_sizes
     K, tvw = 40, 0.1
     log("generating the 64-solve mixed-size queue")
-    draws = config3_sizes(seed=0, count=args.batch_count)
-    probs = [config3_problem(k, size, p, seed=0) for k, (size, p) in enumerate(draws)]
+    draws = mixed_queue_sizes(seed=0, count=args.batch_count)
+    probs = [mixed_queue_problem(k, size, p, seed=0) for k, (size, p) in enumerate(draws)]
     dev = [(to_device(pr.y), to_device(pr.mask_thw), to_device(pr.phi_sum), None)
            for pr in probs]
     pfi = sum(pr.mask_thw.size for pr in probs) * K       # pixel*frame*iter of the queue
@@ -358,16 +358,16 @@ def batched(args):
 
 def other_solves(prob):
     """ADMM-TV on the headline problem (gamma 0.01, the reference default)
-    and the small cfg1 solve (256x256x8, Bernoulli(0.5), K=40, w=0.1): time
+    and the small solve (256x256x8, Bernoulli(0.5), K=40, w=0.1): time
     per solve with data in HBM, and the fused kernel's algorithmic bandwidth."""
     import numpy as np
     import torch
     from pnp_sci import _lib
     from pnp_sci.device import to_device
-    from pnp_sci.synthetic import config1
+    from pnp_sci.synthetic import problem_256x256x8
     out = {}
     cases = (("admm_2048x2048x20_K60", prob, 60, 0.05, _lib.METHOD_ADMM, (5 * 20 + 2) * 4),
-             ("gap_256x256x8_K40", config1(0), 40, 0.1, _lib.METHOD_GAP, gap_bytes_per_pixel(8)))
+             ("gap_256x256x8_K40", problem_256x256x8(0), 40, 0.1, _lib.METHOD_GAP, gap_bytes_per_pixel(8)))
     for name, pr, K, tvw, method, bpp in cases:
         solve = Solver(to_device(pr.y), to_device(pr.mask_thw), to_device(pr.phi_sum), K, tvw)
         solve.p.method = method
@@ -394,11 +394,11 @@ def other_solves(prob):
 
 def unrolled(args):
     import torch
-    from pnp_sci.synthetic import config4, config5
+    from pnp_sci.synthetic import unrolled_batch4_256x256x8, unrolled_batch2_1024x1024x20
     from pnp_sci.unrolled import unrolled_gap_tv
     out = {}
-    for name, (data, K, logits, w0) in (("batch4_256x256x8_K10", (config4, 10, True, 0.1)),
-                                        ("batch2_1024x1024x20_K20", (config5, 20, False, 0.05))):
+    for name, (data, K, logits, w0) in (("batch4_256x256x8_K10", (unrolled_batch4_256x256x8, 10, True, 0.1)),
+                                        ("batch2_1024x1024x20_K20", (unrolled_batch2_1024x1024x20, 20, False, 0.05))):
         truth, mask = data(seed=0)
         tt = torch.tensor(truth, device="cuda")
         m = torch.tensor(mask, device="cuda", requires_grad=logits)
@@ -485,10 +485,10 @@ def cpu_leg(args, prob, solve, want_unrolled):
             checks["oracle_check"] = str(e)
     if want_unrolled:
         import unrolled_ref as R
-        from pnp_sci.synthetic import config5
+        from pnp_sci.synthetic import unrolled_batch2_1024x1024x20
         from pnp_sci.unrolled import unrolled_gap_tv
         log("FP64 reference gradients of the 1024x1024x20 unrolled model")
-        truth, mask = config5(seed=0)
+        truth, mask = unrolled_batch2_1024x1024x20(seed=0)
         Ku = 20
         tt = torch.tensor(truth, device="cuda")
         lt = torch.full((Ku,), 1.0, device="cuda", requires_grad=True)
@@ -534,7 +534,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="2048x2048x20", choices=sorted(CONFIGS))
     ap.add_argument("--cpu-iters", type=int, default=4,
                     help="outer iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -567,7 +567,7 @@ def main():
     prob = make_problem(H, W, T, seed=0, mask=mask_kind, noise_rel=noise)
     head, solve = headline(args, prob)
 
-    if not args.no_traffic and args.config == "cfg2":
+    if not args.no_traffic and args.config == "2048x2048x20":
         try:
             t = measure_traffic()
             head["roofline"]["traffic"] = t["bytes"]
@@ -580,7 +580,7 @@ def main():
         extra["batched_queue"] = batched(args)
     if not args.no_unrolled:
         extra["unrolled"] = unrolled(args)
-    if args.config == "cfg2" and not args.no_batch:
+    if args.config == "2048x2048x20" and not args.no_batch:
         extra["other_solves"] = other_solves(prob)
 
     cpu, checks = cpu_leg(args, prob, solve, want_unrolled=not args.no_unrolled and

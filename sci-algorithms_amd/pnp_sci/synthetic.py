@@ -24,8 +24,9 @@ from dataclasses import dataclass
 
 import numpy as np
 
-__all__ = ["Problem", "truth_video", "make_problem", "config1", "config2", "config3_sizes",
-           "config3_problem"]
+__all__ = ["Problem", "truth_video", "make_problem", "problem_256x256x8", "problem_2048x2048x20",
+           "mixed_queue_sizes", "mixed_queue_problem", "unrolled_batch4_256x256x8",
+           "unrolled_batch2_1024x1024x20"]
 
 
 @dataclass
@@ -106,12 +107,12 @@ def make_problem(H: int, W: int, T: int, *, seed: int = 0, mask: str = "bernoull
     return Problem(truth_thw=truth, mask_thw=phi, y=y, phi_sum=ps.astype(np.float32))
 
 
-def config1(seed: int = 0) -> Problem:
+def problem_256x256x8(seed: int = 0) -> Problem:
     """256x256x8, i.i.d. Bernoulli(0.5) masks, noise-free."""
     return make_problem(256, 256, 8, seed=seed)
 
 
-def config2(seed: int = 0, H: int = 2048, W: int = 2048, T: int = 20) -> Problem:
+def problem_2048x2048x20(seed: int = 0, H: int = 2048, W: int = 2048, T: int = 20) -> Problem:
     """2048x2048x20, shifting Bernoulli(0.5) mask, noise sigma = 0.01*max(y)."""
     return make_problem(H, W, T, seed=seed, mask="shift", noise_rel=0.01)
 
@@ -119,7 +120,7 @@ def config2(seed: int = 0, H: int = 2048, W: int = 2048, T: int = 20) -> Problem
 CONFIG3_SIZES = ((256, 256, 8), (512, 512, 10), (1024, 1024, 20), (512, 768, 16))
 
 
-def config3_sizes(seed: int = 0, count: int = 64):
+def mixed_queue_sizes(seed: int = 0, count: int = 64):
     """Sizes and mask densities of the mixed-size queue (uniform draws)."""
     rng = np.random.default_rng(seed)
     idx = rng.integers(0, len(CONFIG3_SIZES), size=count)
@@ -127,7 +128,7 @@ def config3_sizes(seed: int = 0, count: int = 64):
     return [(CONFIG3_SIZES[i], float(p)) for i, p in zip(idx, ps)]
 
 
-def config3_problem(k: int, size, p: float, seed: int = 0) -> Problem:
+def mixed_queue_problem(k: int, size, p: float, seed: int = 0) -> Problem:
     H, W, T = size
     return make_problem(H, W, T, seed=seed * 1000 + k + 1, p=p)
 
@@ -148,13 +149,13 @@ def unrolled_batch(batch: int, H: int, W: int, T: int, seed: int = 0, mask: str 
     return truth, m
 
 
-def config4(seed: int = 0):
+def unrolled_batch4_256x256x8(seed: int = 0):
     """Unrolled GAP-TV training: batch 4 at 256x256x8, mask logits ~ N(0,1)
     (K=10 outer x 5 TV iterations)."""
     return unrolled_batch(4, 256, 256, 8, seed=seed, mask="logits")
 
 
-def config5(seed: int = 0):
+def unrolled_batch2_1024x1024x20(seed: int = 0):
     """Large unrolled GAP-TV: batch 2 at 1024x1024x20 with a shifting binary
     mask (K=20 outer x 5 TV iterations, learnable lambda_k / tv_weight_k)."""
     return unrolled_batch(2, 1024, 1024, 20, seed=seed, mask="shift")

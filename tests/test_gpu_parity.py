@@ -195,11 +195,11 @@ def test_solve_against_fp64_oracle(gpu, H, W, T, method, kw):
     assert np.allclose(ssim_, want[2], atol=1e-4)
 
 
-def test_config1_full_against_fp64_oracle(gpu):
+def test_256x256x8_full_against_fp64_oracle(gpu):
     """256x256x8, accelerated GAP, 40 iterations, w=0.1, 5 TV iterations."""
     import pnp_sci
-    from pnp_sci.synthetic import config1
-    pr = config1(0)
+    from pnp_sci.synthetic import problem_256x256x8
+    pr = problem_256x256x8(0)
     A64, At64 = ops64(pr.mask)
     A, At = pnp_sci.make_operators(pr.mask)
     x, psnr_, _, pall = pnp_sci.gap_denoise(pr.y, pr.phi_sum, A, At, iter_max=40, tv_weight=0.1,
@@ -208,7 +208,7 @@ def test_config1_full_against_fp64_oracle(gpu):
                          iter_max=40, tv_weight=0.1, tv_iter_max=5,
                          X_orig=pr.truth.astype(np.float64))
     e = rel(x, want[0])
-    print("config1 relative error vs FP64 oracle: %.3g, PSNR %.3f dB (oracle %.3f)"
+    print("problem_256x256x8 relative error vs FP64 oracle: %.3g, PSNR %.3f dB (oracle %.3f)"
           % (e, pall[-1], want[3][-1]))
     assert e <= 1e-4
     assert abs(pall[-1] - want[3][-1]) < 0.01

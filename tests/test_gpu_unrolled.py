@@ -111,20 +111,20 @@ def test_measurement_gradient(gpu):
         assert rel(got[k], want[k]) <= 1e-3, (k, rel(got[k], want[k]))
 
 
-def test_config4_finite_differences(gpu):
+def test_unrolled_batch4_256x256x8_finite_differences(gpu):
     """Batch 4 at 256x256x8, K=10 x 5 TV iterations, mask logits ~ N(0,1):
     the hand-written gradient against FP64 central differences (eps=1e-6) of
     the FP64 reference loss on 32 random unit directions in (logits,
     lambda_k, tv_weight_k), and against FP64 autograd."""
     import torch
-    from pnp_sci.synthetic import config4
-    truth, z = config4(seed=0)
+    from pnp_sci.synthetic import unrolled_batch4_256x256x8
+    truth, z = unrolled_batch4_256x256x8(seed=0)
     K, n = 10, 5
     lam, w = _schedules(K, 0)
     got = _hip(z, truth, lam, w, n, logits=True)
     want = _ref(z, truth, lam, w, n, logits=True)
     errs = {k: rel(got[k], want[k]) for k in ("x", "mask", "lam", "w")}
-    print("config4 relative errors vs FP64 autograd", errs)
+    print("unrolled_batch4_256x256x8 relative errors vs FP64 autograd", errs)
     for k, e in errs.items():
         assert e <= (1e-4 if k == "x" else 1e-3), (k, e)
 
@@ -156,18 +156,18 @@ def test_config4_finite_differences(gpu):
     fds, dirs = np.array(fds), np.array(dirs)
     err = rel(dirs, fds)
     worst = float(np.max(np.abs(dirs - fds) / np.abs(fds)))
-    print("config4 FD check: relative L2 %.3e, worst direction %.3e" % (err, worst))
+    print("unrolled_batch4_256x256x8 FD check: relative L2 %.3e, worst direction %.3e" % (err, worst))
     assert err <= 1e-3
 
 
-def test_config5_checkpointed_gradients(gpu):
+def test_unrolled_batch2_1024x1024x20_checkpointed_gradients(gpu):
     """1024x1024x20, K=20 x 5, batch 2, shifting binary mask, learnable
     lambda_k / tv_weight_k: gradients against the FP64 reference run with
     per-outer-iteration checkpointing."""
     import torch
-    from pnp_sci.synthetic import config5
+    from pnp_sci.synthetic import unrolled_batch2_1024x1024x20
     from pnp_sci.unrolled import unrolled_gap_tv
-    truth, mask = config5(seed=0)
+    truth, mask = unrolled_batch2_1024x1024x20(seed=0)
     K = 20
     lam, w = np.full(K, 1.0, np.float32), np.full(K, 0.05, np.float32)
     tt = torch.tensor(truth, device="cuda")
@@ -181,7 +181,7 @@ def test_config5_checkpointed_gradients(gpu):
     l64 = torch.tensor(lam, **dd, requires_grad=True)
     w64 = torch.tensor(w, **dd, requires_grad=True)
     R.mse_loss(torch.tensor(mask, **dd), tt.double(), l64, w64, checkpoint=True).backward()
-    print("config5 lambda/weight gradient errors", rel(glam, l64.grad.cpu().numpy()),
+    print("unrolled_batch2_1024x1024x20 lambda/weight gradient errors", rel(glam, l64.grad.cpu().numpy()),
           rel(gw, w64.grad.cpu().numpy()))
     assert rel(glam, l64.grad.cpu().numpy()) <= 1e-3
     assert rel(gw, w64.grad.cpu().numpy()) <= 1e-3
