@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(256) k_decide_batch(const FusedArgs* __restric
 
 // Early-exit repair over all (solve, tile) items; returns at once when no
 // frame of the batch stopped early.
-template <int D, int BY, int MR>
+template <int D, int BY, int MR, int MODE>
 __global__ void __launch_bounds__(64 * BY) k_fix_batch(const FusedArgs* __restrict__ descs,
                                                       const int2* __restrict__ items, int n,
                                                       const int* any) {
@@ -100,9 +100,9 @@ __global__ void __launch_bounds__(64 * BY) k_fix_batch(const FusedArgs* __restri
     const FusedArgs& a = descs[it.x];
     if (*a.nexit == 0) continue;
     if (it.y >= 0)
-      fused_fix_tile<D, BY, MR, true>(a, it.y, sh);
+      fused_fix_tile<D, BY, MR, MODE, true>(a, it.y, sh);
     else
-      fused_fix_tile<D, BY, MR, false>(a, ~it.y, sh);
+      fused_fix_tile<D, BY, MR, MODE, false>(a, ~it.y, sh);
     __syncthreads();
   }
 }
@@ -317,8 +317,8 @@ int enqueue_iters(scigap_batch_plan* P, hipStream_t s) {
     SCIGAP_LAUNCH_CHECK("batched fused launch");
     k_decide_batch<D><<<n, 256, 0, s>>>(L.descs + size_t(k) * n, L.any + k);
     SCIGAP_LAUNCH_CHECK("batched decision launch");
-    k_fix_batch<D, BY, 1><<<P->grid, 64 * BY, 0, s>>>(L.descs + size_t(k) * n, L.items, L.nitems,
-                                                      L.any + k);
+    k_fix_batch<D, BY, 1, MODE><<<P->grid, 64 * BY, 0, s>>>(L.descs + size_t(k) * n, L.items,
+                                                            L.nitems, L.any + k);
     SCIGAP_LAUNCH_CHECK("batched repair launch");
   }
   return SCIGAP_OK;

@@ -585,13 +585,13 @@ struct FixLds {
   double red64[BY];
 };
 
-template <int D, int BY, int MR, bool INSIDE>
+template <int D, int BY, int MR, int MODE, bool INSIDE>
 __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, FixLds<D, BY, MR>& sh) {
   using G = TileGeom<D, BY, MR>;
   constexpr int RY = G::RY;
   constexpr int NH = MR;
+  constexpr bool kTvOnly = MODE == TV_ONLY;
   TileLds<BY>& lds = sh.eng;
-  float2* const s_img = sh.img;
   double* const red64 = sh.red64;
   const int H = a.H, W = a.W, T = a.T;
   const int N = H * W;
@@ -600,52 +600,64 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
   const int off0 = tp.off0;
   EdgeMasks em;
   if constexpr (!INSIDE) em.init<RY>(lp, H, W);
-  const bool tv_only = a.mode == TV_ONLY;
+  const bool from_y = a.from_y;
 
-  float q[RY][2];
-  if (tv_only) {
-#pragma unroll
-    for (int j = 0; j < RY; ++j) q[j][0] = q[j][1] = 0.f;
-  } else {
+  // q = lambda*r (or y + lambda*r when X_0 = At(y)), as in the main pass.
+  float q[RY][2] = {};
+  if constexpr (!kTvOnly) {
     load_plane<RY, INSIDE>(a.r_in, q, lp, off0, H, W);
+    if (from_y) {
+      float yv[RY][2];
+      load_plane<RY, INSIDE>(a.y, yv, lp, off0, H, W);
+#pragma unroll
+      for (int j = 0; j < RY; ++j)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) q[j][k] = fmaf(a.lam, q[j][k], yv[j][k]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < RY; ++j)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) q[j][k] *= a.lam;
+    }
   }
-  if (a.from_y) {
-    float yv[RY][2];
-    load_plane<RY, INSIDE>(a.y, yv, lp, off0, H, W);
-#pragma unroll
-    for (int j = 0; j < RY; ++j)
-#pragma unroll
-      for (int k = 0; k < 2; ++k) q[j][k] = fmaf(a.lam, q[j][k], yv[j][k]);
-  } else {
-#pragma unroll
-    for (int j = 0; j < RY; ++j)
-#pragma unroll
-      for (int k = 0; k < 2; ++k) q[j][k] *= a.lam;
-  }
-  float dyb[RY][2];
-#pragma unroll
-  for (int j = 0; j < RY; ++j) dyb[j][0] = dyb[j][1] = 0.f;
+  float dyb[RY][2] = {};
   float err = 0.f;
 
-  for (int t = 0; t < T; ++t) {
+  // The stopped frames are visited in order.  A frame's x and Phi are
+  // requested one stopped frame ahead and its old output before its TV, so
+  // that their latency hides behind the stencil arithmetic (as in the main
+  // pass); nx stays 0 when X_0 = At(y).
+  auto next_stopped = [&](int t) {
+    while (t < T && a.stop[t] >= D) ++t;
+    return t;
+  };
+  int t = next_stopped(0);
+  float nph[RY][2] = {}, nx[RY][2] = {};
+  if (t < T) {
+    if constexpr (!kTvOnly) load_plane<RY, INSIDE>(a.phi + size_t(t) * N, nph, lp, off0, H, W);
+    if (!from_y) load_plane<RY, INSIDE>(a.x_in + size_t(t) * N, nx, lp, off0, H, W);
+  }
+  // Every lane reads back only the LDS slots it wrote itself.
+  float2* const simg = sh.img + lp.w * RY * 64 + lp.lane;
+  while (t < T) {
     const int nup = a.stop[t];
-    if (nup >= D) continue;
-    const float* phit = a.phi + size_t(t) * N;
-    const float* xt = a.x_in + size_t(t) * N;
-    float* xo = a.x_out + size_t(t) * N;
-    float ph[RY][2], xv[RY][2] = {};
-    if (tv_only) {
+    float ph[RY][2];
 #pragma unroll
-      for (int j = 0; j < RY; ++j) ph[j][0] = ph[j][1] = 0.f;
-    } else {
-      load_plane<RY, INSIDE>(phit, ph, lp, off0, H, W);
+    for (int j = 0; j < RY; ++j) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) ph[j][k] = nph[j][k];
+      simg[j * 64] = kTvOnly ? make_float2(nx[j][0], nx[j][1])
+                             : make_float2(fmaf(q[j][0], ph[j][0], nx[j][0]),
+                                           fmaf(q[j][1], ph[j][1], nx[j][1]));
     }
-    if (!a.from_y) load_plane<RY, INSIDE>(xt, xv, lp, off0, H, W);
-    float2* const simg = s_img + lp.w * RY * 64 + lp.lane;
-#pragma unroll
-    for (int j = 0; j < RY; ++j)  // xv = 0 when from_y
-      simg[j * 64] = make_float2(fmaf(q[j][0], ph[j][0], xv[j][0]),
-                                 fmaf(q[j][1], ph[j][1], xv[j][1]));
+    float* xo = a.x_out + size_t(t) * N;
+    float xold[RY][2] = {};
+    if constexpr (!kTvOnly) load_plane<RY, INSIDE>(xo, xold, lp, off0, H, W);
+    const int tn = next_stopped(t + 1);
+    if (tn < T) {
+      if constexpr (!kTvOnly) load_plane<RY, INSIDE>(a.phi + size_t(tn) * N, nph, lp, off0, H, W);
+      if (!from_y) load_plane<RY, INSIDE>(a.x_in + size_t(tn) * N, nx, lp, off0, H, W);
+    }
     float out[RY][2];
     float acc[NH * 8];
     tv_run_from_zero<D, RY, BY, NH, !INSIDE>(simg, out, nup, lds, lp, em, a.weight, a.tw, acc,
@@ -659,28 +671,26 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
         if (!tp.owns(j, k)) continue;
         const unsigned o = unsigned(off0 + j * W + k);
         float xn = out[j][k];
-        const float xold = xo[o];
-        if (!tv_only) {
-          if (a.mode == ADMM) {
-            if (a.clip) xn = fminf(fmaxf(xn, 0.f), 1.f);
-            if (a.write_b) {
-              const float2 im = simg[j * 64];  // this lane's own slot
-              bt[o] = xn - (k ? im.y : im.x);
-            }
-            dyb[j][k] = fmaf(2.f * ph[j][k], xn - xold, dyb[j][k]);
-          } else {
-            if (trt) {  // replace the main pass's error of this pixel
-              float e = trt[o] - xn, eo = trt[o] - xold;
-              err = fmaf(e, e, err);
-              err = fmaf(-eo, eo, err);
-            }
-            dyb[j][k] = fmaf(ph[j][k], xn - xold, dyb[j][k]);
+        if constexpr (MODE == ADMM) {
+          if (a.clip) xn = fminf(fmaxf(xn, 0.f), 1.f);
+          if (a.write_b) {
+            const float2 im = simg[j * 64];  // this lane's own slot
+            bt[o] = xn - (k ? im.y : im.x);
           }
+          dyb[j][k] = fmaf(2.f * ph[j][k], xn - xold[j][k], dyb[j][k]);
+        } else if constexpr (!kTvOnly) {
+          if (trt) {  // replace the main pass's error of this pixel
+            float e = trt[o] - xn, eo = trt[o] - xold[j][k];
+            err = fmaf(e, e, err);
+            err = fmaf(-eo, eo, err);
+          }
+          dyb[j][k] = fmaf(ph[j][k], xn - xold[j][k], dyb[j][k]);
         }
         xo[o] = xn;
       }
+    t = tn;
   }
-  if (!tv_only) {
+  if constexpr (!kTvOnly) {
 #pragma unroll
     for (int j = 0; j < RY; ++j)
 #pragma unroll
@@ -689,18 +699,16 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
         const unsigned o = unsigned(off0 + j * W + k);
         const float yb = a.yb[o] + dyb[j][k];
         a.yb[o] = yb;
-        if (!a.epilogue) continue;
-        if (a.mode == GAP_ACC) epilogue_pixel<GAP_ACC>(a, o, yb);
-        else if (a.mode == GAP_PLAIN) epilogue_pixel<GAP_PLAIN>(a, o, yb);
-        else epilogue_pixel<ADMM>(a, o, yb);
+        if (a.epilogue) epilogue_pixel<MODE>(a, o, yb);
       }
   }
-  if (a.truth && a.err && a.mode != ADMM && !tv_only) commit_err<BY>(err, a.err, red64, lp.w, lp.lane);
+  if constexpr (!kTvOnly && MODE != ADMM)
+    if (a.truth && a.err) commit_err<BY>(err, a.err, red64, lp.w, lp.lane);
 }
 
 // For D <= 4 at least 4 waves per SIMD: two 8-wave workgroups per CU
 // (128 VGPRs); deeper tiles need more registers than that.
-template <int D, int BY, int MR>
+template <int D, int BY, int MR, int MODE>
 __global__ void __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(D <= 4 ? 4 : 1)))
 k_fused_fix(FusedArgs a) {
   __shared__ FixLds<D, BY, MR> sh;
@@ -709,9 +717,9 @@ k_fused_fix(FusedArgs a) {
   if (tile >= a.ntiles) return;
   // Interior tiles run without masks or bounds checks (as in the main pass).
   if (tile_inside<D, BY, MR>(tile, a.tiles_x, a.H, a.W))
-    fused_fix_tile<D, BY, MR, true>(a, tile, sh);
+    fused_fix_tile<D, BY, MR, MODE, true>(a, tile, sh);
   else
-    fused_fix_tile<D, BY, MR, false>(a, tile, sh);
+    fused_fix_tile<D, BY, MR, MODE, false>(a, tile, sh);
 }
 
 // Initial residual: X_0 = x0 (or At(y)), yb = A(X_0), then

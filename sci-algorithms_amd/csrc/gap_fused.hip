@@ -230,7 +230,12 @@ int launch_fused(const FusedArgs& a0, hipStream_t s, int tile_rows, int engine) 
   SCIGAP_LAUNCH_CHECK("early-exit decision launch");
   {
     ProfScope ps("fused_fix", s);
-    k_fused_fix<D, FBY, MR><<<grid, 64 * FBY, 0, s>>>(a);
+    switch (a.mode) {
+      case GAP_ACC: k_fused_fix<D, FBY, MR, GAP_ACC><<<grid, 64 * FBY, 0, s>>>(a); break;
+      case GAP_PLAIN: k_fused_fix<D, FBY, MR, GAP_PLAIN><<<grid, 64 * FBY, 0, s>>>(a); break;
+      case ADMM: k_fused_fix<D, FBY, MR, ADMM><<<grid, 64 * FBY, 0, s>>>(a); break;
+      default: k_fused_fix<D, FBY, MR, TV_ONLY><<<grid, 64 * FBY, 0, s>>>(a); break;
+    }
   }
   SCIGAP_LAUNCH_CHECK("fused early-exit launch");
   return SCIGAP_OK;
