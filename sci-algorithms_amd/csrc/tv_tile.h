@@ -99,22 +99,33 @@ __device__ __forceinline__ void tv_div(const float2* img, float (&out)[RY][2],
   // Publish this wave's last row of p0 for the wave below.
   float2* xp = reinterpret_cast<float2*>(lds.xp);
   xp[lp.w * 64 + lp.lane] = make_float2(p0[RY - 1][0], p0[RY - 1][1]);
+  // The img slots are this lane's own: read them before the barrier, so that
+  // their latency overlaps the wait instead of following it.
+  float2 im[RY];
+#pragma unroll
+  for (int j = 0; j < RY; ++j) im[j] = img[j * 64];
   __syncthreads();
   float2 above = lp.w > 0 ? xp[(lp.w - 1) * 64 + lp.lane] : make_float2(0.f, 0.f);
-  float s[NH] = {};
+  // Rows 1.. first: only row 0 needs the row of the wave above, whose read
+  // is still in flight.  The squares are summed in row order afterwards.
+  float d0[RY], d1[RY];
 #pragma unroll
-  for (int j = 0; j < RY; ++j) {
+  for (int jj = 0; jj < RY; ++jj) {
+    const int j = jj + 1 < RY ? jj + 1 : 0;
     float up0 = j > 0 ? p0[j - 1][0] : above.x;
     float up1 = j > 0 ? p0[j - 1][1] : above.y;
     float left0 = from_prev_lane(p1[j][1]);
-    float d0 = (up0 - p0[j][0]) + (left0 - p1[j][0]);
-    float d1 = (up1 - p0[j][1]) + (p1[j][0] - p1[j][1]);
-    const float2 im = img[j * 64];
-    out[j][0] = im.x + d0;
-    out[j][1] = im.y + d1;
+    d0[j] = (up0 - p0[j][0]) + (left0 - p1[j][0]);
+    d1[j] = (up1 - p0[j][1]) + (p1[j][0] - p1[j][1]);
+    out[j][0] = im[j].x + d0[j];
+    out[j][1] = im[j].y + d1[j];
+  }
+  float s[NH] = {};
+#pragma unroll
+  for (int j = 0; j < RY; ++j) {
     const int h = (NH == 2 && j >= RY / 2) ? 1 : 0;
-    s[h] = fmaf(d0, d0, s[h]);
-    s[h] = fmaf(d1, d1, s[h]);
+    s[h] = fmaf(d0[j], d0[j], s[h]);
+    s[h] = fmaf(d1[j], d1[j], s[h]);
   }
   if (dacc)
 #pragma unroll
