@@ -243,7 +243,7 @@ struct FusedLds {
   static constexpr int LH = TileGeom<D, BY, MR>::LH;
   std::conditional_t<X1, TileLdsX1<BY>, TileLds<BY>> eng;
   float2 img[LH * 64];
-  float red[kEnergyFrames * BY * D];  // per-frame, per-wave energy partials
+  float red[kEnergyFrames * BY * EnergySlots<D>::kPerWave];  // per-frame, per-wave energy partials
   double red64[BY];
   std::conditional_t<DMA, DmaPlanes<LH>, NoDmaPlanes> dma;
 };
@@ -343,7 +343,8 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
 #pragma unroll
     for (int j = 0; j < RY; ++j) yb[j][0] = yb[j][1] = 0.f;
   }
-  for (int i = threadIdx.x; i < kEnergyFrames * BY * D; i += blockDim.x) red[i] = 0.f;
+  for (int i = threadIdx.x; i < kEnergyFrames * BY * EnergySlots<D>::kPerWave; i += blockDim.x)
+    red[i] = 0.f;
   float err = 0.f;
 
   // Frame t+1's x and Phi are requested before frame t's TV so that their
@@ -467,7 +468,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
       }
     }
     const int tb = t % kEnergyFrames;
-    energy_add<D, BY, MR, NH>(acc, tp.lane_owner, red + tb * BY * D, lp);
+    energy_add<D, BY, MR, NH>(acc, tp.lane_owner, red + tb * BY * EnergySlots<D>::kPerWave, lp);
     if (tb == kEnergyFrames - 1 || t == T - 1) {
       energy_flush<D, BY>(red, tb + 1, en - size_t(tb) * D);
     }

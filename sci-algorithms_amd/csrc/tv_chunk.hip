@@ -58,7 +58,7 @@ template <int D, int BY>
 struct ChunkLds {
   TileLds<BY> eng;
   float2 img[TileGeom<D, BY, 1>::LH * 64];
-  float red[kEnergyFrames * BY * D];
+  float red[kEnergyFrames * BY * EnergySlots<D>::kPerWave];
 };
 
 // Engine run from a loaded dual field: out = img + d(p), then nup updates.
@@ -99,7 +99,8 @@ __device__ __forceinline__ void chunk_tile(const ChunkArgs& a, int tile, ChunkLd
   if constexpr (!INSIDE) em.init<RY>(lp, H, W);
   float2* const img = sh.img + lp.w * RY * 64 + lp.lane;
   if constexpr (!REPAIR)
-    for (int i = threadIdx.x; i < kEnergyFrames * BY * D; i += blockDim.x) sh.red[i] = 0.f;
+    for (int i = threadIdx.x; i < kEnergyFrames * BY * EnergySlots<D>::kPerWave; i += blockDim.x)
+      sh.red[i] = 0.f;
   for (int t = 0; t < T; ++t) {
     const int nup = REPAIR ? a.fix[t] : (a.stop[t] < 0 ? a.nup : -1);  // uniform
     if (nup >= 0) {
@@ -139,21 +140,21 @@ __device__ __forceinline__ void chunk_tile(const ChunkArgs& a, int tile, ChunkLd
         float e[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) e[i] = acc[i];
-        energy_add<D, BY, 1, 1>(e, tp.lane_owner, sh.red + (t % kEnergyFrames) * BY * D, lp);
+        energy_add<D, BY, 1, 1>(e, tp.lane_owner,
+                                sh.red + (t % kEnergyFrames) * BY * EnergySlots<D>::kPerWave, lp);
       }
     }
     if constexpr (!REPAIR) {
       const int tb = t % kEnergyFrames;
       if (tb == kEnergyFrames - 1 || t == T - 1) {
-        // Strided flush into E[t][i0 + u], u < nup (slots [frame][wave][u]).
+        // Strided flush into E[t][i0 + u], u < nup.
         // Slots of frames skipped in this block keep older values; those
         // frames have stopped and their energies are never read again.
         __syncthreads();
         const int t0 = t - tb;
         for (int i = threadIdx.x; i < (tb + 1) * D; i += blockDim.x) {
           const int tl = i / D, u = i - tl * D;
-          float sum = 0.f;
-          for (int w = 0; w < BY; ++w) sum += sh.red[(tl * BY + w) * D + u];
+          const float sum = energy_block_sum<D, BY>(sh.red, tl, u);
           if (u < a.nup) atomicAdd(a.energy + size_t(t0 + tl) * a.ne + a.i0 + u, double(sum));
         }
         __syncthreads();

@@ -364,21 +364,43 @@ __device__ __forceinline__ void tv_run_x1(const float2* img, float (&out)[RY][2]
   }
 }
 
-// Frame energies: each wave reduces its lanes' partials to wave-uniform sums
-// (DPP and lane reads) and lane 0 writes them to the wave's slots of the
-// frame, [frame % kEnergyFrames][wave][update] floats of LDS, so frames need
-// no barrier and no atomics; energy_flush sums the waves of a block of frames
-// in a fixed order and adds the totals to the global fp64 accumulators (one
-// barrier pair and T*D atomics per workgroup per block).  Halo waves hold no
-// interior rows and never write: their slots keep the zeros written at the
-// start of the tile.  A lane contributes half h of its strip when those rows
-// are interior rows of the tile and its columns are interior columns.
+// Frame energies.  Each wave reduces its lanes' partials and writes the
+// result to its own LDS slots of the frame ([frame % kEnergyFrames][wave]
+// [slot] floats), so frames need no barrier and no atomics; energy_flush sums
+// the waves of a block of frames in a fixed order and adds the totals to the
+// global fp64 accumulators (one barrier pair and T*D atomics per workgroup
+// per block).  Halo waves hold no interior rows and never write: their slots
+// keep the zeros written at the start of the tile.  A lane contributes half h
+// of its strip when those rows are interior rows of the tile and its columns
+// are interior columns.
+//
+// D <= 4 (transposed row sums): within each quad of lanes the D <= 4 partials
+// are exchanged so that lane q of the quad ends with the quad's sum of one
+// update, e(q) = 2*(q&1) + (q>>1) (three selects-and-adds); two row rotations
+// then sum the four quads of each 16-lane row, and lanes 0-3 of every row
+// write their update's row sum: 16 slots per wave, [row][update].  D > 4: one
+// wave-uniform sum per update (DPP within rows, then lane reads), D slots.
 constexpr int kEnergyFrames = 32;  // frames buffered in LDS between flushes
+
+template <int D>
+struct EnergySlots {
+  static constexpr bool kRows = D <= 4;
+  static constexpr int kRowsPerWave = kRows ? 4 : 1;
+  static constexpr int kPerWave = kRows ? 16 : D;  // floats per wave and frame
+  __host__ __device__ static constexpr int slot(int row, int u) { return kRows ? row * 4 + u : u; }
+};
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xf,
+                                                            0xf, true));
+}
 
 template <int D, int BY, int MR, int NH>
 __device__ __forceinline__ void energy_add(const float (&acc)[NH * 8], bool lane_owner,
-                                           float* frame_slots /* LDS, BY*D floats */,
+                                           float* frame_slots /* LDS, BY * kPerWave floats */,
                                            const LanePos& lp) {
+  using ES = EnergySlots<D>;
   bool lo_in, hi_in;
   if (MR == 2) {
     lo_in = lp.w > 0;          // rows [0, D) of wave 0 are the top halo
@@ -387,32 +409,70 @@ __device__ __forceinline__ void energy_add(const float (&acc)[NH * 8], bool lane
     lo_in = hi_in = lp.w > 0 && lp.w < BY - 1;
   }
   if (!lo_in && !hi_in) return;  // wave-uniform: halo waves hold no interior rows
-  float s[D];
+  constexpr int P = ES::kRows ? 4 : D;
+  float s[P];
 #pragma unroll
-  for (int u = 0; u < D; ++u) {
+  for (int u = 0; u < P; ++u) {
     float v = 0.f;
-    if (lane_owner) {
+    if (u < D && lane_owner) {
       v = lo_in ? acc[u] : 0.f;
       if (NH == 2) v += hi_in ? acc[8 + u] : 0.f;
     }
-    s[u] = wave_sum_uniform(v);
+    s[u] = v;
   }
-  if (lp.lane == 0)
+  if constexpr (ES::kRows) {
+    const int l = lp.lane;
+    const bool b0 = l & 1, b1 = (l >> 1) & 1;
+    // Quad lanes 2i and 2i+1 swap halves: even lanes keep updates {0, 1},
+    // odd lanes {2, 3} (quad_perm [1,0,3,2]) ...
+    float a0 = b0 ? s[2] : s[0], c0 = b0 ? s[0] : s[2];
+    float a1 = b0 ? s[3] : s[1], c1 = b0 ? s[1] : s[3];
+    a0 += dpp_move<0xB1>(c0);
+    a1 += dpp_move<0xB1>(c1);
+    // ... then lanes i and i^2 (quad_perm [2,3,0,1]): lane q keeps update
+    // 2*(q&1) + (q>>1), summed over the quad.
+    float k = b1 ? a1 : a0;
+    const float c = b1 ? a0 : a1;
+    k += dpp_move<0x4E>(c);
+    // The four quads of a row hold the same updates at the same quad
+    // positions: row_ror 4, then row_ror 8.
+    k += dpp_move<0x124>(k);
+    k += dpp_move<0x128>(k);
+    const int q = l & 15;
+    const int e = 2 * (q & 1) + (q >> 1);
+    if (q < 4 && e < D) frame_slots[lp.w * ES::kPerWave + ES::slot(l >> 4, e)] = k;
+  } else {
+    float r[D];
 #pragma unroll
-    for (int u = 0; u < D; ++u) frame_slots[lp.w * D + u] = s[u];
+    for (int u = 0; u < D; ++u) r[u] = wave_sum_uniform(s[u]);
+    if (lp.lane == 0)
+#pragma unroll
+      for (int u = 0; u < D; ++u) frame_slots[lp.w * ES::kPerWave + u] = r[u];
+  }
 }
 
-// Add frames [t0, t0+nt) of the LDS block ([frame][wave][update]) to energy
-// (fp64, [T][D] from frame t0).  All threads of the workgroup call it.
+// Energy of update u of frame t in an LDS block of frames: the sum of every
+// wave's slots, in a fixed order.
+template <int D, int BY>
+__device__ __forceinline__ float energy_block_sum(const float* block, int t, int u) {
+  using ES = EnergySlots<D>;
+  float s = 0.f;
+  for (int w = 0; w < BY; ++w) {
+    const float* ws = block + (t * BY + w) * ES::kPerWave;
+#pragma unroll
+    for (int r = 0; r < ES::kRowsPerWave; ++r) s += ws[ES::slot(r, u)];
+  }
+  return s;
+}
+
+// Add frames [t0, t0+nt) of the LDS block to energy (fp64, [T][D] from frame
+// t0).  All threads of the workgroup call it.
 template <int D, int BY>
 __device__ __forceinline__ void energy_flush(const float* block, int nt, double* energy_t0) {
   __syncthreads();
   for (int i = threadIdx.x; i < nt * D; i += blockDim.x) {
     const int t = i / D, u = i - t * D;
-    float s = 0.f;
-#pragma unroll 4
-    for (int w = 0; w < BY; ++w) s += block[(t * BY + w) * D + u];
-    atomicAdd(energy_t0 + i, double(s));
+    atomicAdd(energy_t0 + i, double(energy_block_sum<D, BY>(block, t, u)));
   }
   __syncthreads();
 }
