@@ -254,6 +254,53 @@ def test_forced_early_exit_inside_fused_solve(gpu):
         assert rel(thw_to_hwt(x).cpu().numpy(), want) <= 1e-4, method
 
 
+@pytest.mark.parametrize("case", [
+    # (H, W, T, method, accelerate, tv_iter_max, iters, tv_weight, tile_rows, engine, p)
+    (151, 130, 8, "gap", True, 7, 26, 0.02, 0, 0, 0.5),
+    (70, 2, 8, "admm", True, 7, 9, 0.02, 0, 0, 0.5),
+    (141, 127, 11, "admm", True, 5, 20, 0.05, 16, 0, 0.5),
+    (115, 33, 11, "gap", True, 5, 20, 0.02, 12, 6, 0.5),
+    (129, 250, 6, "gap", False, 5, 30, 0.02, 8, 6, 0.6),
+])
+def test_natural_early_exits_against_fp64_oracle(gpu, case):
+    """Frames that stop early at the default tolerance, scattered over the
+    frames and iterations of a solve (the repair visits only the stopped
+    frames and prefetches the next one): the solve still matches the FP64
+    oracle, and early exits did happen."""
+    import torch
+    from pnp_sci import _lib
+    from pnp_sci.device import to_device
+    H, W, T, method, acc, n, K, w, rows, engine, pm = case
+    L = gpu
+    pr = problem(H, W, T, seed=H + W + T, p=pm)
+    A64, At64 = ops64(pr.mask)
+    y64, ps64 = pr.y.astype(np.float64), pr.phi_sum.astype(np.float64)
+    if method == "gap":
+        want = O.gap_denoise(y64, ps64, A64, At64, accelerate=acc, iter_max=K, tv_weight=w,
+                             tv_iter_max=n, show_iqa=False)[0]
+    else:
+        want = O.admm_denoise(y64, ps64, A64, At64, iter_max=K, tv_weight=w, tv_iter_max=n,
+                              show_iqa=False)[0]
+    p = _lib.Params()
+    L.scigap_default_params(p, H, W, T)
+    p.method = _lib.METHOD_ADMM if method == "admm" else _lib.METHOD_GAP
+    p.accelerate = 1 if acc else 0
+    p.iters, p.tv_weight, p.tv_iter_max = K, w, n
+    p.tile_rows, p.engine = rows, engine
+    y, phi, ps = to_device(pr.y), to_device(pr.mask_thw), to_device(pr.phi_sum)
+    nb = L.scigap_solve_workspace_bytes(p)
+    ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    x = torch.empty((T, H, W), device="cuda")
+    stats = torch.zeros(K, dtype=torch.int32, device="cuda")
+    _lib.check(L.scigap_solve(p, y.data_ptr(), phi.data_ptr(), ps.data_ptr(), None, x.data_ptr(),
+                              None, None, stats.data_ptr(), ws.data_ptr(), nb, _lib.stream_ptr()))
+    exits = int(stats.sum().item())
+    e = rel(x.cpu().numpy().transpose(1, 2, 0), want)
+    print(case, "early exits", exits, "rel %.2e" % e)
+    assert exits > 0, "no early exit was exercised"
+    assert e <= 1e-4
+
+
 def test_zero_mask_pixels_and_errors(gpu):
     import pnp_sci
     pr = problem(33, 47, 5, seed=5)
