@@ -92,19 +92,6 @@ __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcp
 __device__ __forceinline__ void compute_here(float& v) { asm volatile("" : "+v"(v)); }
 __device__ __forceinline__ void compute_here(unsigned& v) { asm volatile("" : "+v"(v)); }
 
-// Sum over the 64 lanes of a wavefront, result valid in every lane: DPP
-// within each 16-lane row (pairs, quads, half-row and row mirrors), then two
-// cross-row exchanges.
-__device__ __forceinline__ float wave_sum_dpp(float v) {
-  v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xf, 0xf, true));
-  v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x4E, 0xf, 0xf, true));
-  v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x141, 0xf, 0xf, true));
-  v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x140, 0xf, 0xf, true));
-  v += __shfl_xor(v, 16, 64);
-  v += __shfl_xor(v, 32, 64);
-  return v;
-}
-
 // Sum over the 64 lanes of a wavefront as a wave-uniform value: DPP within
 // each 16-lane row (every lane of a row ends with the row's sum), then the
 // four row sums read from lanes 0, 16, 32, 48 (no LDS round trips).
