@@ -12,7 +12,7 @@ from __future__ import annotations
 import ctypes
 
 from . import _lib
-from .device import hwt_to_thw, thw_to_hwt, to_device, to_host_like
+from .device import thw_to_hwt, to_device, to_device_thw, to_host_like
 
 __all__ = ["BatchPlan", "gap_denoise_batch"]
 
@@ -102,8 +102,8 @@ def gap_denoise_batch(ys, masks, phi_sums, *, x0s=None, method="gap", iter_max=5
     probs = []
     for i, (y, m, ps) in enumerate(zip(ys, masks, phi_sums)):
         x0 = x0s[i] if x0s is not None and x0s[i] is not None else None
-        probs.append((to_device(y), hwt_to_thw(to_device(m)), to_device(ps),
-                      hwt_to_thw(to_device(x0)) if x0 is not None else None))
+        probs.append((to_device(y), to_device_thw(m), to_device(ps),
+                      to_device_thw(x0) if x0 is not None else None))
     plan = BatchPlan(probs, method=method, iters=iter_max, _lambda=_lambda, accelerate=accelerate,
                      gamma=gamma, clip=clip, tv_weight=tv_weight, tv_iter_max=tv_iter_max,
                      use_graph=use_graph)

@@ -27,7 +27,7 @@ import time
 import numpy as np
 
 from . import _lib
-from .device import hwt_to_thw, is_torch, thw_to_hwt, to_device, to_host_like
+from .device import hwt_to_thw, is_torch, thw_to_hwt, to_device, to_device_thw, to_host_like
 from .metrics import psnr_frames, ssim_frames
 
 __all__ = ["gap_denoise", "admm_denoise", "admmdenoise_cacti", "GAP_TV_rec", "ADMM_TV_rec",
@@ -61,17 +61,17 @@ def recover_mask(A, At, y, check: bool = True):
     if len(np.shape(mask_hwt)) != 3 or tuple(np.shape(mask_hwt)[:2]) != tuple(np.shape(y)):
         raise ValueError("At(ones) must return an (H,W,T) mask, got shape %s"
                          % (tuple(np.shape(mask_hwt)),))
-    phi = hwt_to_thw(to_device(mask_hwt, "At(ones)"))
+    phi = to_device_thw(mask_hwt, "At(ones)")
     if check:
         H, W, T = np.shape(mask_hwt)
         rng = np.random.default_rng(12345)
-        probe = rng.random((H, W, T)).astype(np.float32)
+        probe = rng.random((H, W, T), dtype=np.float32)
         if is_torch(y):
             probe_in = torch.from_numpy(probe).to(device=y.device, dtype=y.dtype)
         else:
-            probe_in = probe.astype(np.asarray(y).dtype)
+            probe_in = probe.astype(np.asarray(y).dtype, copy=False)
         got = to_device(A(probe_in), "A(probe)")
-        pd = hwt_to_thw(to_device(probe, "probe"))
+        pd = to_device_thw(probe, "probe")
         want = torch.empty((H, W), dtype=torch.float32, device=pd.device)
         L = _lib.lib()
         _lib.check(L.scigap_forward(pd.data_ptr(), phi.data_ptr(), want.data_ptr(), H, W, T,
@@ -195,9 +195,9 @@ def _run(method, y, Phi_sum, A, At, *, _lambda, accelerate, gamma, clip, denoise
     psd = to_device(Phi_sum, "Phi_sum")
     if tuple(yd.shape) != (H, W) or tuple(psd.shape) != (H, W):
         raise ValueError("y and Phi_sum must be (H,W) = (%d,%d)" % (H, W))
-    x0d = hwt_to_thw(to_device(x0, "x0")) if x0 is not None else None
+    x0d = to_device_thw(x0, "x0") if x0 is not None else None
     want_iqa = show_iqa and X_orig is not None
-    truth = hwt_to_thw(to_device(X_orig, "X_orig")) if X_orig is not None else None
+    truth = to_device_thw(X_orig, "X_orig") if X_orig is not None else None
     x, sq, _ = solve_device(yd, phi, psd, method=method, iters=iters, _lambda=_lambda,
                             accelerate=accelerate, gamma=gamma, clip=clip, tv_weight=tv_weight,
                             tv_iter_max=tv_iter_max, tv_eps=tv_eps, x0=x0d,
@@ -392,7 +392,7 @@ class _MaskOnly:
     """Operator stand-in carrying an explicit (H,W,T) mask."""
 
     def __init__(self, Phi):
-        self.scigap_phi = hwt_to_thw(to_device(Phi, "Phi"))
+        self.scigap_phi = to_device_thw(Phi, "Phi")
 
     def __call__(self, v):  # pragma: no cover
         raise RuntimeError("mask-only operator is not callable")

@@ -11,7 +11,7 @@ from __future__ import annotations
 import math
 
 from . import _lib
-from .device import hwt_to_thw, thw_to_hwt, to_device, to_host_like, sq_error
+from .device import thw_to_hwt, to_device, to_device_thw, to_host_like, sq_error
 
 __all__ = ["A_", "At_", "psnr", "make_operators", "SCIOperator"]
 
@@ -24,8 +24,8 @@ def _torch():
 def A_(x, Phi):
     """y[h,w] = sum_t x[h,w,t]*Phi[h,w,t]  (utils.py:10-15) via scigap_forward."""
     torch = _torch()
-    xd = hwt_to_thw(to_device(x, "x"))
-    pd = hwt_to_thw(to_device(Phi, "Phi"))
+    xd = to_device_thw(x, "x")
+    pd = to_device_thw(Phi, "Phi")
     T, H, W = pd.shape
     if tuple(xd.shape) != (T, H, W):
         raise ValueError("x and Phi shapes differ: %s vs %s" % (tuple(x.shape), tuple(Phi.shape)))
@@ -40,7 +40,7 @@ def At_(y, Phi):
     """x[h,w,t] = y[h,w]*Phi[h,w,t]  (utils.py:17-26) via scigap_adjoint."""
     torch = _torch()
     yd = to_device(y, "y")
-    pd = hwt_to_thw(to_device(Phi, "Phi"))
+    pd = to_device_thw(Phi, "Phi")
     T, H, W = pd.shape
     if tuple(yd.shape) != (H, W):
         raise ValueError("y shape %s does not match Phi %s" % (tuple(y.shape), tuple(Phi.shape)))
@@ -76,7 +76,7 @@ class SCIOperator:
 
     def __init__(self, mask, adjoint: bool):
         self.mask_hwt = mask
-        self.scigap_phi = hwt_to_thw(to_device(mask, "mask"))
+        self.scigap_phi = to_device_thw(mask, "mask")
         self.adjoint = adjoint
 
     def __call__(self, v):

@@ -61,6 +61,42 @@ def test_forward_adjoint_phisum(gpu, shape):
         O.psnr(x.astype(np.float64), y[..., None] * np.ones(T)), abs=1e-4)
 
 
+def test_staged_host_transfers_round_trip(gpu):
+    """Large NumPy arrays cross PCIe through the pinned staging buffers (several
+    chunks and a ragged tail); the result equals NumPy's own float32 cast bit for
+    bit, whatever the input dtype or memory order."""
+    import torch
+    from pnp_sci import device
+    rng = np.random.default_rng(3)
+    n = 3 * (device._STAGE_BYTES // 4) + 12345
+    cases = [
+        rng.random(n, dtype=np.float32),
+        rng.standard_normal(n),                                      # float64 -> float32
+        rng.random(n) < 0.5,                                         # bool
+        rng.random((n // 97, 97), dtype=np.float32).T,               # non-contiguous view
+        rng.random(1000, dtype=np.float32),                          # below the staging size
+    ]
+    for a in cases:
+        d = device.to_device(a)
+        assert d.dtype == torch.float32 and tuple(d.shape) == a.shape
+        want = np.ascontiguousarray(a, dtype=np.float32)
+        assert np.array_equal(d.cpu().numpy(), want)
+        back = device.to_host_like(d, want)
+        assert back.dtype == np.float32 and np.array_equal(back, want)
+        assert np.array_equal(device.to_host_like(d * 2, np.zeros(1)), (want * 2).astype(np.float64))
+    # (H,W,T) volumes in every memory order reach the device frame-major
+    H, W, T = 301, 517, 9
+    thw = rng.random((T, H, W), dtype=np.float32)
+    want = torch.from_numpy(thw)
+    for hwt in (np.ascontiguousarray(thw.transpose(1, 2, 0)),   # C order
+                thw.transpose(1, 2, 0),                          # view of frame-major data
+                np.asfortranarray(thw.transpose(1, 2, 0)),       # MATLAB / loadmat order
+                thw.transpose(1, 2, 0)[:, ::2, :]):              # strided
+        got = device.to_device_thw(hwt).cpu()
+        assert got.is_contiguous() and torch.equal(got, torch.from_numpy(
+            np.ascontiguousarray(hwt.transpose(2, 0, 1))))
+
+
 def test_ssim_frames_match_oracle(gpu):
     from pnp_sci.device import hwt_to_thw, to_device
     from pnp_sci.metrics import ssim_frames
