@@ -100,9 +100,9 @@ __global__ void __launch_bounds__(64 * BY) k_fix_batch(const FusedArgs* __restri
     const FusedArgs& a = descs[it.x];
     if (*a.nexit == 0) continue;
     if (it.y >= 0)
-      fused_fix_tile<D, BY, MR, MODE, true>(a, it.y, sh);
+      fused_fix_tile<D, BY, MR, MODE, true>(a, it.y, sh, a.stop);
     else
-      fused_fix_tile<D, BY, MR, MODE, false>(a, ~it.y, sh);
+      fused_fix_tile<D, BY, MR, MODE, false>(a, ~it.y, sh, a.stop);
     __syncthreads();
   }
 }

@@ -587,7 +587,8 @@ struct FixLds {
 };
 
 template <int D, int BY, int MR, int MODE, bool INSIDE>
-__device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, FixLds<D, BY, MR>& sh) {
+__device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, FixLds<D, BY, MR>& sh,
+                                               const int* __restrict__ stop) {
   using G = TileGeom<D, BY, MR>;
   constexpr int RY = G::RY;
   constexpr int NH = MR;
@@ -629,7 +630,7 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
   // that their latency hides behind the stencil arithmetic (as in the main
   // pass); nx stays 0 when X_0 = At(y).
   auto next_stopped = [&](int t) {
-    while (t < T && a.stop[t] >= D) ++t;
+    while (t < T && stop[t] >= D) ++t;
     return t;
   };
   int t = next_stopped(0);
@@ -641,7 +642,7 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
   // Every lane reads back only the LDS slots it wrote itself.
   float2* const simg = sh.img + lp.w * RY * 64 + lp.lane;
   while (t < T) {
-    const int nup = a.stop[t];
+    const int nup = stop[t];
     float ph[RY][2];
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
@@ -707,20 +708,50 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
     if (a.truth && a.err) commit_err<BY>(err, a.err, red64, lp.w, lp.lane);
 }
 
+// Solves of up to kDecideFrames frames need no separate decision launch:
+// every repair workgroup derives the exit iterations itself from the T*D
+// energies (into LDS), and workgroup 0 publishes them (stop, nexit, stats)
+// as k_decide would.  Longer solves run k_decide first.
+constexpr int kDecideFrames = 1024;
+
 // For D <= 4 at least 4 waves per SIMD: two 8-wave workgroups per CU
 // (128 VGPRs); deeper tiles need more registers than that.
 template <int D, int BY, int MR, int MODE>
 __global__ void __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(D <= 4 ? 4 : 1)))
 k_fused_fix(FusedArgs a) {
   __shared__ FixLds<D, BY, MR> sh;
-  if (*a.nexit == 0) return;  // the common case: one scalar load per workgroup
+  __shared__ int s_stop[kDecideFrames];
+  __shared__ int s_count;
+  const int* stop = a.stop;
+  if (a.T <= kDecideFrames) {
+    if (threadIdx.x == 0) s_count = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int t = threadIdx.x; t < a.T; t += blockDim.x) {
+      const int st = exit_iteration<D>(a.energy + size_t(t) * D, a.eps);
+      s_stop[t] = st;
+      if (blockIdx.x == 0) a.stop[t] = st;
+      mine += st < D;
+    }
+    if (mine) atomicAdd(&s_count, mine);
+    __syncthreads();
+    const int count = s_count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      *a.nexit = count;
+      if (a.stats) *a.stats = count;
+    }
+    if (count == 0) return;  // the common case
+    stop = s_stop;
+  } else if (*a.nexit == 0) {
+    return;  // decided by k_decide: one scalar load per workgroup
+  }
   const int tile = tile_of_block(blockIdx.x, a.per_xcd);
   if (tile >= a.ntiles) return;
   // Interior tiles run without masks or bounds checks (as in the main pass).
   if (tile_inside<D, BY, MR>(tile, a.tiles_x, a.H, a.W))
-    fused_fix_tile<D, BY, MR, MODE, true>(a, tile, sh);
+    fused_fix_tile<D, BY, MR, MODE, true>(a, tile, sh, stop);
   else
-    fused_fix_tile<D, BY, MR, MODE, false>(a, tile, sh);
+    fused_fix_tile<D, BY, MR, MODE, false>(a, tile, sh, stop);
 }
 
 // Initial residual: X_0 = x0 (or At(y)), yb = A(X_0), then

@@ -223,11 +223,11 @@ int launch_fused(const FusedArgs& a0, hipStream_t s, int tile_rows, int engine) 
   FusedArgs a = a0;
   constexpr int FBY = FixShape<D>::BY;
   const unsigned grid = set_tiles<D, FBY, MR>(a);
-  {
+  if (a.T > kDecideFrames) {  // otherwise the repair kernel decides
     ProfScope ps("fused_decide", s);
     k_decide<D><<<1, 256, 0, s>>>(a.energy, a.T, a.eps, a.stop, a.nexit, a.stats);
+    SCIGAP_LAUNCH_CHECK("early-exit decision launch");
   }
-  SCIGAP_LAUNCH_CHECK("early-exit decision launch");
   {
     ProfScope ps("fused_fix", s);
     switch (a.mode) {
