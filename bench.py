@@ -202,20 +202,51 @@ def measure_traffic():
 # Batched mixed-size queue (64 solves) against one-at-a-time solves
 # ---------------------------------------------------------------------------
 
-def batched(args):
-    import numpy as np
-    import torch
-    from pnp_sci.batch import BatchPlan
+def _mixed_queue(args):
+    """The mixed-size queue (sizes drawn from {256x256x8, 512x512x10, 1024x1024x20,
+    512x768x16}, Bernoulli(p~U(0.3,0.7)) masks) on the device."""
     from pnp_sci.device import to_device
     from pnp_sci.synthetic import mixed_queue_problem, mixed_queue_sizes
-    K, tvw = 40, 0.1
-    log("generating the 64-solve mixed-size queue")
+    log("generating the %d-solve mixed-size queue" % args.batch_count)
     draws = mixed_queue_sizes(seed=0, count=args.batch_count)
     probs = [mixed_queue_problem(k, size, p, seed=0) for k, (size, p) in enumerate(draws)]
     dev = [(to_device(pr.y), to_device(pr.mask_thw), to_device(pr.phi_sum), None)
            for pr in probs]
-    pfi = sum(pr.mask_thw.size for pr in probs) * K       # pixel*frame*iter of the queue
     bytes_it = sum(gap_bytes_per_pixel(pr.mask_thw.shape[0]) * pr.y.size for pr in probs)
+    return draws, probs, dev, bytes_it
+
+
+def _time_plan(plan, reps=3):
+    import torch
+    plan.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.run()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def batched_queue_only(args):
+    """The batched queue alone (no comparators), for profiling its kernels."""
+    from pnp_sci.batch import BatchPlan
+    K, tvw = 40, 0.1
+    _draws, _probs, dev, bytes_it = _mixed_queue(args)
+    plan = BatchPlan(dev, iters=K, tv_weight=tvw, use_graph=True)
+    t = _time_plan(plan)
+    plan.close()
+    return {"solves": len(dev), "outer_iters": K, "ms": t * 1e3,
+            "algorithmic_GBps": bytes_it * K / t / 1e9,
+            "mode": "persistent tile queue, hipGraph replay"}
+
+
+def batched(args):
+    import numpy as np
+    import torch
+    from pnp_sci.batch import BatchPlan
+    K, tvw = 40, 0.1
+    draws, probs, dev, bytes_it = _mixed_queue(args)
+    pfi = sum(pr.mask_thw.size for pr in probs) * K       # pixel*frame*iter of the queue
 
     singles = [Solver(y, phi, ps, K, tvw) for (y, phi, ps, _) in dev]
     for s in singles:
@@ -305,14 +336,7 @@ def batched(args):
     t_torch = time.perf_counter() - t0
 
     plan = BatchPlan(dev, iters=K, tv_weight=tvw, use_graph=True)
-    plan.run()
-    torch.cuda.synchronize()
-    reps = 3
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        plan.run()
-    torch.cuda.synchronize()
-    t_batch = (time.perf_counter() - t0) / reps
+    t_batch = _time_plan(plan)
 
     def mean_psnr(xs):
         v = []
@@ -547,6 +571,9 @@ def main():
     ap.add_argument("--no-unrolled", action="store_true", help="skip the unrolled model")
     ap.add_argument("--only-unrolled", action="store_true",
                     help="run only the unrolled-model leg and print its JSON (for profiling)")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run only the batched queue (no comparators) and print its JSON "
+                         "(for profiling)")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
 
@@ -559,6 +586,9 @@ def main():
         return
     if args.only_unrolled:
         print(json.dumps({"unrolled": unrolled(args)}), flush=True)
+        return
+    if args.only_batch:
+        print(json.dumps({"batched_queue": batched_queue_only(args)}), flush=True)
         return
 
     from pnp_sci.synthetic import make_problem

@@ -574,7 +574,8 @@ __global__ void __launch_bounds__(256) k_decide(const double* __restrict__ energ
   }
 }
 
-// Early-exit repair (does work only when k_decide found an early stop).
+// Early-exit repair (does work only when some frame stopped early; `stop`
+// holds each frame's exit iteration, D for none).
 // For every frame that stopped at s < D, the frame is recomputed with s
 // updates from the unchanged inputs (x_in and r_in are double-buffered), its
 // contribution to yb is corrected by Phi*(x_new - x_old) (ADMM: 2*Phi*...),
