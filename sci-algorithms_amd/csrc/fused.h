@@ -34,9 +34,9 @@ struct FusedArgs {
   float* b;             // ADMM b (T,H,W)
   double* energy;       // [T][D] energies of this iteration
   float* yb;            // (H,W) sum_t Phi_t*x_t^{new} on owned pixels (for the repair)
-  int* stop;            // [T] exit iteration per frame (written by k_decide);
+  int* stop;            // [T] exit iteration per frame (k_decide or the repair kernel);
                         // null: no energy test (eps = 0), no decision/repair
-  int* nexit;           // frames of this iteration that stopped early (k_decide)
+  int* nexit;           // frames of this iteration that stopped early (same writers)
   int* stats;           // optional copy of *nexit for the caller
   int H, W, T;
   float lam, gam_next, weight, tw;
