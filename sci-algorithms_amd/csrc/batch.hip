@@ -114,12 +114,13 @@ __global__ void __launch_bounds__(256) k_init_batch(const FusedArgs* __restrict_
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
        i += int64_t(gridDim.x) * blockDim.x) {
     const float yv = a.y[i];
-    float yb = 0.f;
+    double sum = 0.0;
     for (int t = 0; t < a.T; ++t) {
       const float f = a.phi[t * N + i];
       const float xv = a.from_y ? yv * f : a.x_in[t * N + i];
-      yb = fmaf(xv, f, yb);
+      sum = acc_prod(xv, f, sum);
     }
+    const float yb = float(sum);
     const float s = a.phi_sum[i];
     float* r = const_cast<float*>(a.r_in);
     if (a.mode == GAP_ACC) {

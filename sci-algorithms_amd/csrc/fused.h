@@ -322,7 +322,12 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
 
   const bool from_y = a.from_y;
   // q = lambda*r (or y + lambda*r when X_0 = At(y)) over the load tile.
-  float q[RY][2], yb[RY][2];
+  float q[RY][2];
+  // A-sum accumulators: fp64 for D <= 4 (115 VGPRs, two workgroups per CU);
+  // fp32 for the deeper tiles, whose extra registers would cost a resident
+  // workgroup (D = 5: 476 -> 662 us per 2048x2048x20 launch).
+  using YbAcc = std::conditional_t<(D <= 4), double, float>;
+  YbAcc yb[RY][2];
   if constexpr (MODE != TV_ONLY) {
     load_plane<RY, INSIDE>(a.r_in, q, lp, off0, H, W);
     if (from_y) {
@@ -341,7 +346,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
   }
   if constexpr (MODE != TV_ONLY) {
 #pragma unroll
-    for (int j = 0; j < RY; ++j) yb[j][0] = yb[j][1] = 0.f;
+    for (int j = 0; j < RY; ++j) yb[j][0] = yb[j][1] = YbAcc(0);
   }
   for (int i = threadIdx.x; i < kEnergyFrames * BY * EnergySlots<D>::kPerWave; i += blockDim.x)
     red[i] = 0.f;
@@ -445,7 +450,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
               }
             }
             bv[j][k] = xn - v;
-            yb[j][k] = fmaf(ph[j][k], 2.f * xn - v, yb[j][k]);
+            yb[j][k] = acc_prod(ph[j][k], 2.f * xn - v, yb[j][k]);
           } else if constexpr (MODE != TV_ONLY) {
             if constexpr (IQA) {
               if (tp.owns(j, k)) {
@@ -453,7 +458,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
                 err = fmaf(e, e, err);
               }
             }
-            yb[j][k] = fmaf(ph[j][k], xn, yb[j][k]);
+            yb[j][k] = acc_prod(ph[j][k], xn, yb[j][k]);
           }
           out[j][k] = xn;
         }
@@ -488,7 +493,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
       for (int k = 0; k < 2; ++k) {
         if (!tp.own_c[k]) continue;
         const unsigned o = unsigned(off0 + j * W + k);
-        const float v = yb[j][k];
+        const float v = float(yb[j][k]);
         if (nparts > 1) {
           // the parts are summed (and the epilogue run) by k_part_epilogue
           st1(a.yb_part + size_t(part) * N, N, o, v);
@@ -509,8 +514,9 @@ template <int MODE>
 __global__ void __launch_bounds__(256) k_part_epilogue(FusedArgs a) {
   const int N = a.H * a.W;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-    float yb = a.yb_part[i];
-    for (int f = 1; f < a.fsplit; ++f) yb += a.yb_part[size_t(f) * N + i];
+    double sum = a.yb_part[i];
+    for (int f = 1; f < a.fsplit; ++f) sum += a.yb_part[size_t(f) * N + i];
+    const float yb = float(sum);
     if (a.yb) a.yb[i] = yb;
     if (a.epilogue) epilogue_pixel<MODE>(a, unsigned(i), yb);
   }
@@ -700,6 +706,7 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
       for (int k = 0; k < 2; ++k) {
         if (!tp.owns(j, k)) continue;
         const unsigned o = unsigned(off0 + j * W + k);
+        // The corrections are small differences: their fp32 sum is exact enough.
         const float yb = a.yb[o] + dyb[j][k];
         a.yb[o] = yb;
         if (a.epilogue) epilogue_pixel<MODE>(a, o, yb);
@@ -766,12 +773,14 @@ __global__ void __launch_bounds__(256) k_init(const float* __restrict__ x0,
                                               int64_t N, int T, int mode, float gam) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
        i += int64_t(gridDim.x) * blockDim.x) {
-    float yv = y[i], yb = 0.f;
+    const float yv = y[i];
+    double sum = 0.0;
     for (int t = 0; t < T; ++t) {
       float f = phi[t * N + i];
       float xv = x0 ? x0[t * N + i] : yv * f;
-      yb = fmaf(xv, f, yb);
+      sum = acc_prod(xv, f, sum);
     }
+    const float yb = float(sum);
     float s = phi_sum[i];
     if (mode == GAP_ACC) {
       float v1 = yv - yb;
@@ -827,7 +836,7 @@ __global__ void __launch_bounds__(256) k_project(FusedArgs a) {
        i += int64_t(gridDim.x) * blockDim.x) {
     const float lr = a.lam * a.r_in[i];
     const float yv = a.y[i];
-    float yb = 0.f;
+    double yb = 0.0;
     for (int t = 0; t < a.T; ++t) {
       const size_t o = size_t(t) * N + i;
       const float f = a.phi[o];
@@ -841,20 +850,20 @@ __global__ void __launch_bounds__(256) k_project(FusedArgs a) {
           err = fmaf(e, e, err);
         }
         if (a.write_b) a.b[o] = xn - v;
-        yb = fmaf(f, 2.f * xn - v, yb);
+        yb = acc_prod(f, 2.f * xn - v, yb);
       } else {
         if (a.truth) {
           float e = a.truth[o] - xn;
           err = fmaf(e, e, err);
         }
-        yb = fmaf(f, xn, yb);
+        yb = acc_prod(f, xn, yb);
       }
       a.x_out[o] = xn;
     }
     if (a.epilogue) {
-      if (a.mode == GAP_ACC) epilogue_pixel<GAP_ACC>(a, unsigned(i), yb);
-      else if (a.mode == GAP_PLAIN) epilogue_pixel<GAP_PLAIN>(a, unsigned(i), yb);
-      else epilogue_pixel<ADMM>(a, unsigned(i), yb);
+      if (a.mode == GAP_ACC) epilogue_pixel<GAP_ACC>(a, unsigned(i), float(yb));
+      else if (a.mode == GAP_PLAIN) epilogue_pixel<GAP_PLAIN>(a, unsigned(i), float(yb));
+      else epilogue_pixel<ADMM>(a, unsigned(i), float(yb));
     }
   }
   if (a.truth && a.err) {
@@ -887,7 +896,7 @@ __global__ void __launch_bounds__(256) k_gap_epi(FusedArgs a) {
   float err = 0.f;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
        i += int64_t(gridDim.x) * blockDim.x) {
-    float yb = 0.f;
+    double yb = 0.0;
     for (int t = 0; t < a.T; ++t) {
       const size_t o = size_t(t) * N + i;
       const float f = a.phi[o];
@@ -903,19 +912,19 @@ __global__ void __launch_bounds__(256) k_gap_epi(FusedArgs a) {
           err = fmaf(e, e, err);
         }
         if (a.write_b) a.b[o] = xn - v;
-        yb = fmaf(f, 2.f * xn - v, yb);
+        yb = acc_prod(f, 2.f * xn - v, yb);
       } else {
         if (a.truth) {
           float e = a.truth[o] - xn;
           err = fmaf(e, e, err);
         }
-        yb = fmaf(f, xn, yb);
+        yb = acc_prod(f, xn, yb);
       }
     }
     if (a.epilogue) {
-      if (a.mode == GAP_ACC) epilogue_pixel<GAP_ACC>(a, unsigned(i), yb);
-      else if (a.mode == GAP_PLAIN) epilogue_pixel<GAP_PLAIN>(a, unsigned(i), yb);
-      else epilogue_pixel<ADMM>(a, unsigned(i), yb);
+      if (a.mode == GAP_ACC) epilogue_pixel<GAP_ACC>(a, unsigned(i), float(yb));
+      else if (a.mode == GAP_PLAIN) epilogue_pixel<GAP_PLAIN>(a, unsigned(i), float(yb));
+      else epilogue_pixel<ADMM>(a, unsigned(i), float(yb));
     }
   }
   if (a.truth && a.err) {

@@ -69,12 +69,14 @@ __global__ void __launch_bounds__(256) k_ub_proj(
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
        i += int64_t(gridDim.x) * blockDim.x) {
     const float ri = r[i], si = S[i];
-    float rbar = 0.f, fsum = 0.f;
+    double rsum = 0.0;
+    float fsum = 0.f;
     for (int t = 0; t < T; ++t) {
       const float vb = vbar[t * N + i], f = phi[t * N + i];
-      rbar = fmaf(vb, f, rbar);
+      rsum = acc_prod(vb, f, rsum);
       fsum += f;
     }
+    float rbar = float(rsum);
     lsum = fmaf(rbar, ri, lsum);  // dL/dlambda = sum_t vbar*r*Phi
     rbar *= lam;
     const float y1tot = (y1bar_ext ? y1bar_ext[i] : 0.f) + rbar / si;
@@ -102,14 +104,14 @@ __global__ void __launch_bounds__(256) k_ub_init(const float* __restrict__ xbar,
                                                  float* __restrict__ phibar, int64_t N, int T) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
        i += int64_t(gridDim.x) * blockDim.x) {
-    float acc = 0.f;
+    double acc = 0.0;
     const float yv = y[i];
     for (int t = 0; t < T; ++t) {
       const size_t o = size_t(t) * N + i;
-      acc = fmaf(xbar[o], phi[o], acc);
+      acc = acc_prod(xbar[o], phi[o], acc);
       if (phibar) phibar[o] = fmaf(xbar[o], yv, phibar[o]);
     }
-    ybar[i] += acc;
+    ybar[i] += float(acc);
   }
 }
 

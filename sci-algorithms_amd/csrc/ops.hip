@@ -39,7 +39,7 @@ inline unsigned grid_for(int64_t n, int per_thread = 1) {
 }
 
 // y = sum_t x_t * phi_t.  One thread handles 4 consecutive pixels (N % 4 == 0)
-// or 1 pixel; the frame loop keeps the running sum in registers.
+// or 1 pixel; the frame loop keeps the running sums in registers (fp64).
 template <int V>
 __global__ void __launch_bounds__(kThreads) k_forward(const float* __restrict__ x,
                                                       const float* __restrict__ phi,
@@ -48,20 +48,20 @@ __global__ void __launch_bounds__(kThreads) k_forward(const float* __restrict__ 
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < nv;
        i += int64_t(gridDim.x) * blockDim.x) {
     if constexpr (V == 4) {
-      float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
       for (int t = 0; t < T; ++t) {
         float4 a = reinterpret_cast<const float4*>(x + t * N)[i];
         float4 m = reinterpret_cast<const float4*>(phi + t * N)[i];
-        acc.x = fmaf(a.x, m.x, acc.x);
-        acc.y = fmaf(a.y, m.y, acc.y);
-        acc.z = fmaf(a.z, m.z, acc.z);
-        acc.w = fmaf(a.w, m.w, acc.w);
+        s0 = acc_prod(a.x, m.x, s0);
+        s1 = acc_prod(a.y, m.y, s1);
+        s2 = acc_prod(a.z, m.z, s2);
+        s3 = acc_prod(a.w, m.w, s3);
       }
-      reinterpret_cast<float4*>(y)[i] = acc;
+      reinterpret_cast<float4*>(y)[i] = make_float4(float(s0), float(s1), float(s2), float(s3));
     } else {
-      float acc = 0.f;
-      for (int t = 0; t < T; ++t) acc = fmaf(x[t * N + i], phi[t * N + i], acc);
-      y[i] = acc;
+      double acc = 0.0;
+      for (int t = 0; t < T; ++t) acc = acc_prod(x[t * N + i], phi[t * N + i], acc);
+      y[i] = float(acc);
     }
   }
 }

@@ -92,6 +92,18 @@ __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcp
 __device__ __forceinline__ void compute_here(float& v) { asm volatile("" : "+v"(v)); }
 __device__ __forceinline__ void compute_here(unsigned& v) { asm volatile("" : "+v"(v)); }
 
+// Per-pixel sums over the frames (A x = sum_t Phi_t x_t and its corrections)
+// are accumulated in fp64 from the exact products of the fp32 factors and
+// rounded to fp32 once.  A sequential fp32 sum loses about T ulps, which the
+// residual y - A x then amplifies: at T = 1000 a solve would drift 3.6e-4 from
+// the FP64 oracle, against 8e-6 with this accumulation (NumPy's pairwise fp32
+// sum: 1.1e-5).  FP64 FMA issues at the FP32 rate on this GPU.
+__device__ __forceinline__ double acc_prod(float a, float b, double acc) {
+  return fma(double(a), double(b), acc);
+}
+// The same step for an fp32 accumulator (register-bound kernels, see their use).
+__device__ __forceinline__ float acc_prod(float a, float b, float acc) { return fmaf(a, b, acc); }
+
 // Sum over the 64 lanes of a wavefront as a wave-uniform value: DPP within
 // each 16-lane row (every lane of a row ends with the row's sum), then the
 // four row sums read from lanes 0, 16, 32, 48 (no LDS round trips).

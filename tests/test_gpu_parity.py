@@ -168,6 +168,64 @@ def test_tv_early_exit_matches_oracle(gpu, n):
     assert early >= 1
 
 
+def test_more_than_1024_frames(gpu, monkeypatch):
+    """Past 1024 frames the exit decisions come from the separate decision
+    kernel rather than from the repair kernel itself: a TV call whose frames
+    stop at different iterations, and a short solve, still match the FP64
+    oracle (exit iterations equal, values within the usual bounds).  The solve
+    uses eps = 1, so every frame stops at the first exit test in FP32 and FP64
+    alike (with 63-pixel frames, exit tests at the default eps sit close enough
+    to their threshold that the two arithmetics can decide differently)."""
+    import torch
+    from pnp_sci import _lib
+    from pnp_sci.device import hwt_to_thw, thw_to_hwt, to_device
+    from pnp_sci.tv import tv_device
+    rng = np.random.default_rng(11)
+    T, H, W = 1100, 6, 9
+    r, c = np.arange(H)[:, None], np.arange(W)[None, :]
+    img = np.stack([0.5 + 0.2 * np.cos(2 * np.pi * (r / H + c / (2 * W) + t / 37.0)) if t % 2 == 0
+                    else rng.random((H, W)) for t in range(T)], -1).astype(np.float32)
+    out, stop = tv_device(hwt_to_thw(to_device(img)), 0.1, 2e-2, 5, return_stop=True)
+    got, stops = thw_to_hwt(out).cpu().numpy(), stop.cpu().tolist()
+    early = 0
+    for t in range(T):
+        st = {}
+        want = O.tv_chambolle_nd(img[..., t].astype(np.float64), 0.1, 2e-2, 5, _stats=st)
+        assert stops[t] == st["returned"], t
+        early += st["returned"] < 4
+        assert np.max(np.abs(got[..., t] - want)) <= 2e-5
+    assert 0 < early < T
+    H, W, T, K = 9, 7, 1030, 4
+    pr = problem(H, W, T, seed=6)
+    A64, At64 = ops64(pr.mask)
+    calls = []
+    tv64 = O.tv_chambolle_nd
+
+    def counting_tv(image, weight=0.1, eps=2e-4, n_iter_max=200, _stats=None):
+        st = {}
+        res = tv64(image, weight, eps, n_iter_max, _stats=st)
+        calls.append(st["returned"] < n_iter_max - 1)
+        return res
+    monkeypatch.setattr(O, "tv_chambolle_nd", counting_tv)
+    want = O.gap_denoise(pr.y.astype(np.float64), pr.phi_sum.astype(np.float64), A64, At64,
+                         iter_max=K, tv_weight=0.1, show_iqa=False, _tv_eps=1.0)[0]
+    want_exits = np.array(calls).reshape(K, T).sum(1).tolist()
+    assert want_exits == [T] * K
+    L = gpu
+    p = _lib.Params()
+    L.scigap_default_params(p, H, W, T)
+    p.iters, p.tv_weight, p.tv_eps = K, 0.1, 1.0
+    y, phi, ps = to_device(pr.y), to_device(pr.mask_thw), to_device(pr.phi_sum)
+    nb = L.scigap_solve_workspace_bytes(p)
+    ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    x = torch.empty((T, H, W), device="cuda")
+    stats = torch.zeros(K, dtype=torch.int32, device="cuda")
+    _lib.check(L.scigap_solve(p, y.data_ptr(), phi.data_ptr(), ps.data_ptr(), None, x.data_ptr(),
+                              None, None, stats.data_ptr(), ws.data_ptr(), nb, _lib.stream_ptr()))
+    assert stats.cpu().tolist() == want_exits
+    assert rel(x.cpu().numpy().transpose(1, 2, 0), want) <= 1e-4
+
+
 # ---------------------------------------------------------------------------
 # GAP-TV / ADMM-TV solves
 # ---------------------------------------------------------------------------
