@@ -237,6 +237,11 @@ struct DmaPlanes {  // next frame's x and Phi rows, [tile row][128] floats
   float ph[LH * 128];
 };
 struct NoDmaPlanes {};
+template <int LH>
+struct YbSlots {
+  double v[LH * 64 * 2];
+};
+struct NoYbSlots {};
 
 template <int D, int BY, int MR, bool X1 = false, bool DMA = false>
 struct FusedLds {
@@ -246,6 +251,8 @@ struct FusedLds {
   float red[kEnergyFrames * BY * EnergySlots<D>::kPerWave];  // per-frame, per-wave energy partials
   double red64[BY];
   std::conditional_t<DMA, DmaPlanes<LH>, NoDmaPlanes> dma;
+  // fp64 A-sum accumulators of the deeper tiles, [tile row][lane][column]
+  std::conditional_t<(D > 4), YbSlots<LH>, NoYbSlots> ybs;
 };
 
 // Direct-to-LDS prefetch of RY rows (RY even) of one plane for this wave:
@@ -323,11 +330,18 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
   const bool from_y = a.from_y;
   // q = lambda*r (or y + lambda*r when X_0 = At(y)) over the load tile.
   float q[RY][2];
-  // A-sum accumulators: fp64 for D <= 4 (115 VGPRs, two workgroups per CU);
-  // fp32 for the deeper tiles, whose extra registers would cost a resident
-  // workgroup (D = 5: 476 -> 662 us per 2048x2048x20 launch).
-  using YbAcc = std::conditional_t<(D <= 4), double, float>;
-  YbAcc yb[RY][2];
+  // fp64 A-sum accumulators: registers for D <= 4 (115 VGPRs, two workgroups
+  // per CU); for the deeper tiles this lane's LDS slots, because fp64
+  // registers would cost a resident workgroup (D = 5: 476 -> 662 us per
+  // 2048x2048x20 launch).  There an fp32 partial over each block of
+  // kYbBlock frames (about kYbBlock ulps) is added to the slot without a
+  // return value, once per block.
+  constexpr bool kYbInLds = D > 4;
+  constexpr int kYbBlock = 8;  // frames per fp32 partial before an LDS add
+  double yb[RY][2];
+  float ybp[RY][2];  // fp32 partial over the current block of frames
+  double* ybl = nullptr;
+  if constexpr (kYbInLds) ybl = sh.ybs.v + size_t(sl0) * 2;  // row j at ybl[j * 128]
   if constexpr (MODE != TV_ONLY) {
     load_plane<RY, INSIDE>(a.r_in, q, lp, off0, H, W);
     if (from_y) {
@@ -346,7 +360,15 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
   }
   if constexpr (MODE != TV_ONLY) {
 #pragma unroll
-    for (int j = 0; j < RY; ++j) yb[j][0] = yb[j][1] = YbAcc(0);
+    for (int j = 0; j < RY; ++j) {
+      if constexpr (kYbInLds) {
+        ybl[j * 128] = 0.0;
+        ybl[j * 128 + 1] = 0.0;
+        ybp[j][0] = ybp[j][1] = 0.f;
+      } else {
+        yb[j][0] = yb[j][1] = 0.0;
+      }
+    }
   }
   for (int i = threadIdx.x; i < kEnergyFrames * BY * EnergySlots<D>::kPerWave; i += blockDim.x)
     red[i] = 0.f;
@@ -450,7 +472,10 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
               }
             }
             bv[j][k] = xn - v;
-            yb[j][k] = acc_prod(ph[j][k], 2.f * xn - v, yb[j][k]);
+            if constexpr (kYbInLds)
+              ybp[j][k] = fmaf(ph[j][k], 2.f * xn - v, ybp[j][k]);
+            else
+              yb[j][k] = acc_prod(ph[j][k], 2.f * xn - v, yb[j][k]);
           } else if constexpr (MODE != TV_ONLY) {
             if constexpr (IQA) {
               if (tp.owns(j, k)) {
@@ -458,7 +483,10 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
                 err = fmaf(e, e, err);
               }
             }
-            yb[j][k] = acc_prod(ph[j][k], xn, yb[j][k]);
+            if constexpr (kYbInLds)
+              ybp[j][k] = fmaf(ph[j][k], xn, ybp[j][k]);
+            else
+              yb[j][k] = acc_prod(ph[j][k], xn, yb[j][k]);
           }
           out[j][k] = xn;
         }
@@ -470,6 +498,17 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
         if constexpr (MODE == ADMM) {
           if (a.write_b) store_owned<D, BY, MR, INSIDE>(bt, N, W, tp, base, bv);
         }
+      }
+    }
+    if constexpr (kYbInLds && MODE != TV_ONLY) {
+      if (owns_any && (t % kYbBlock == kYbBlock - 1 || t == T - 1)) {  // wave-uniform
+#pragma unroll
+        for (int j = 0; j < RY; ++j)
+#pragma unroll
+          for (int k = 0; k < 2; ++k) {
+            atomicAdd(&ybl[j * 128 + k], double(ybp[j][k]));
+            ybp[j][k] = 0.f;
+          }
       }
     }
     const int tb = t % kEnergyFrames;
@@ -493,7 +532,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
       for (int k = 0; k < 2; ++k) {
         if (!tp.own_c[k]) continue;
         const unsigned o = unsigned(off0 + j * W + k);
-        const float v = float(yb[j][k]);
+        const float v = kYbInLds ? float(ybl[j * 128 + k]) : float(yb[j][k]);
         if (nparts > 1) {
           // the parts are summed (and the epilogue run) by k_part_epilogue
           st1(a.yb_part + size_t(part) * N, N, o, v);
