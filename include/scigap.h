@@ -9,6 +9,8 @@
  *
  * Conventions
  *  - All array pointers are DEVICE pointers owned by the caller, fp32.
+ *    Arithmetic is fp32; sums over frames (A x = sum_t Phi_t x_t) are
+ *    accumulated in fp64 and rounded to fp32 once.
  *  - Video volumes are frame-major (T,H,W), C order: element (t,h,w) is at
  *    t*H*W + h*W + w.  The reference holds (H,W,T); scigap_hwt_to_thw /
  *    scigap_thw_to_hwt convert on the device.
@@ -54,7 +56,7 @@ const char* scigap_last_error(void);
 /* Forward model                                                          */
 /* ---------------------------------------------------------------------- */
 
-/* y[h,w] = sum_t x[t,h,w] * phi[t,h,w].
+/* y[h,w] = sum_t x[t,h,w] * phi[t,h,w] (fp64 sum of exact products, one rounding).
  * Replaces A_(x, Phi), PnP_SCI/python/utils.py:10-15. */
 int scigap_forward(const float* x, const float* phi, float* y,
                    int64_t H, int64_t W, int64_t T, void* stream);
