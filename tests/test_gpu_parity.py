@@ -224,6 +224,15 @@ def test_more_than_1024_frames(gpu, monkeypatch):
                               None, None, stats.data_ptr(), ws.data_ptr(), nb, _lib.stream_ptr()))
     assert stats.cpu().tolist() == want_exits
     assert rel(x.cpu().numpy().transpose(1, 2, 0), want) <= 1e-4
+    # ADMM sums Phi_t (2 theta_t - v_t) over the same 1030 frames
+    want = O.admm_denoise(pr.y.astype(np.float64), pr.phi_sum.astype(np.float64), A64, At64,
+                          iter_max=K, tv_weight=0.1, show_iqa=False, _tv_eps=1.0)[0]
+    p.method = _lib.METHOD_ADMM
+    nb = L.scigap_solve_workspace_bytes(p)
+    ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    _lib.check(L.scigap_solve(p, y.data_ptr(), phi.data_ptr(), ps.data_ptr(), None, x.data_ptr(),
+                              None, None, None, ws.data_ptr(), nb, _lib.stream_ptr()))
+    assert rel(x.cpu().numpy().transpose(1, 2, 0), want) <= 1e-4
 
 
 # ---------------------------------------------------------------------------
