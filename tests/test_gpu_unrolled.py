@@ -79,6 +79,7 @@ CASES = [
     (1, 150, 270, 3, 2, 6, True),    # depth 5, interior and border backward tiles
     (2, 100, 200, 3, 2, 4, False),   # depth 3
     (2, 5, 9, 2, 2, 5, True),        # frames smaller than the reverse tile's halo
+    (1, 9, 7, 600, 2, 5, False),     # 600 frames: the frame sums of forward and reverse
 ]
 
 
