@@ -26,23 +26,36 @@ Sources restated (paths relative to the reference checkout):
   (``_denoise_tv_chambolle_nd``) and ``:396-479`` (``denoise_tv_chambolle``),
   and ``skimage/metrics/simple_metrics.py`` ``peak_signal_noise_ratio``.
 
-How each function is pinned (the tests under ``tests/`` do the checking):
+How each function is pinned (the tests under ``tests/`` do the checking;
+fixtures under ``tests/golden/`` are written by scikit-image 0.18.3 itself,
+``make_tv_golden.py``):
 
 * ``A_``/``At_``/``phi_sum``: closed forms; adjointness <A x, y> = <x, At y>,
-  At(1) = Phi, A(At(y)) = y * sum_t Phi_t^2.
+  At(1) = Phi, A(At(y)) = y * sum_t Phi_t^2, a zero-mask pixel gets sum 1.
 * ``tv_chambolle_nd``/``denoise_tv_chambolle``: bit-exact float32 and float64
-  golden vectors written by scikit-image 0.18.3 (``tests/golden/``), plus the
-  invariants n_iter_max=1 -> identity, constant frame unchanged, frame mean
-  preserved, one-update closed form, and the early-exit branch.
-* ``gap_denoise``/``admm_denoise``: golden end-to-end vectors computed with
-  the reference's loop structure around the scikit-image TV, plus: plain GAP
-  with an identity denoiser is an exact projection (A x = y where Phi_sum > 0);
-  the first accelerated step equals the plain step; ADMM with gamma=0 and
-  an identity denoiser reproduces plain GAP.
+  golden vectors (127 cases), plus the invariants n_iter_max=1 -> identity,
+  constant frame unchanged, frame mean preserved, the w -> inf one-update
+  limit img + Laplacian/4 (scipy.ndimage.laplace), and the early-exit branch.
+* ``gap_denoise``/``admm_denoise``/``admm_denoise_clip``: bit-exact golden
+  solves (the reference's loops around scikit-image's TV), plus: plain GAP
+  with an identity denoiser is an exact projection (A x = y where
+  Phi_sum > 0); the first accelerated step equals the plain step; ADMM with
+  gamma=0 and an identity denoiser reproduces plain GAP.
+* ``GAP_TV_rec``/``ADMM_TV_rec``: bit-exact golden solves with scikit-image's
+  30-iteration TV and the 0.999/0.998 decays (``legacy_golden.npz``); the
+  fixtures are checked to contain early TV exits and to reject a missing or
+  misplaced decay; GAP_TV_rec also meets ``gap_denoise`` on the float64 ones.
+* ``admmdenoise_cacti``: every block equals the direct solve, with the warm
+  start and the result (and its PSNR/SSIM lists) reversed on the odd ('updown')
+  or even ('downup') coded frames counted from ``iframe``, and MAXB scaling.
 * ``gap_denoise_bayer``: equals four independent ``gap_denoise`` runs on the
   2x2 sub-grids reassembled into the mosaic (the TV is per frame, so the
   joint multichannel TV decouples); mosaic PSNRs equal the closed form.
-* ``psnr``: closed form (``utils.py:28-36``), 100 for a zero error.
+* ``psnr``: scikit-image ``peak_signal_noise_ratio`` values with data_range 1
+  (``psnr_golden.npz``; equal in float64), closed forms (20 dB at mse 0.01,
+  40 dB at 1e-4), 100 for a zero error (``utils.py:28-36``).
+* ``compare_psnr``: the same scikit-image values to 1e-12 in float32 and
+  float64, inf for a zero error, and the data_range scaling.
 * ``compare_ssim``: golden values written by scikit-image 0.18.3.
 """
 
@@ -96,10 +109,16 @@ def psnr(ref, img):
 def compare_psnr(ref, img, data_range=1.0):
     """scikit-image ``peak_signal_noise_ratio`` as called at pnp_sci_algo.py:702.
 
-    10*log10(data_range^2 / mse), computed in float64 (scikit-image casts
-    both images to float64 before the error); inf for a zero error.
+    10*log10(data_range^2 / mse).  As scikit-image 0.18.3 does
+    (``metrics/simple_metrics.py`` ``_as_floats`` and ``mean_squared_error``),
+    both images are promoted to ``result_type(ref, img, float32)``, the
+    difference is squared in that type and the mean is accumulated in
+    float64; inf for a zero error.
     """
-    err = np.mean((np.asarray(ref, np.float64) - np.asarray(img, np.float64)) ** 2)
+    ref, img = np.asarray(ref), np.asarray(img)
+    ft = np.result_type(ref.dtype, img.dtype, np.float32)
+    d = np.asarray(ref, ft) - np.asarray(img, ft)
+    err = np.mean(d ** 2, dtype=np.float64)
     if err == 0:
         return math.inf
     return 10 * math.log10(data_range ** 2 / err)

@@ -65,26 +65,17 @@ __global__ void __launch_bounds__(64 * BY) k_fused_batch(const FusedArgs* __rest
   }
 }
 
-// Early-exit decision for every solve of the batch (one workgroup each);
-// *any counts the stopped frames of the whole batch.
+// Early-exit decision for every frame of every solve of the batch (one
+// workgroup per (solve, frame): grid n x max T); *any counts the stopped
+// frames of the whole batch.
 template <int D>
 __global__ void __launch_bounds__(256) k_decide_batch(const FusedArgs* __restrict__ descs,
                                                       int* any) {
   const FusedArgs& a = descs[blockIdx.x];
-  __shared__ int cnt;
-  if (threadIdx.x == 0) cnt = 0;
-  __syncthreads();
-  for (int t = threadIdx.x; t < a.T; t += blockDim.x) {
-    const int s = exit_iteration<D>(a.energy + size_t(t) * D, a.eps);
-    a.stop[t] = s;
-    if (s < D) atomicAdd(&cnt, 1);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    *a.nexit = cnt;
-    if (a.stats) *a.stats = cnt;
-    if (cnt) atomicAdd(any, cnt);
-  }
+  const int t = blockIdx.y;
+  if (t >= a.T) return;
+  decide_frame<D>(a, t);
+  if (threadIdx.x == 0 && a.stop[t] < D) atomicAdd(any, 1);
 }
 
 // Early-exit repair over all (solve, tile) items; returns at once when no
@@ -188,9 +179,10 @@ struct BatchLayout {
   int2* items = nullptr;        // all items (repair pass)
   int* counters = nullptr;      // [fused_iters][8]
   int* any = nullptr;           // [fused_iters]
-  double* energy = nullptr;     // all solves' energies, contiguous
-  size_t energy_count = 0;
+  int* zeroed = nullptr;        // zeroed before every run: per solve nexit [K], then alt[0] [T]
+  int zeroed_count = 0;
   int nitems = 0;
+  int maxT = 0;
   size_t bytes = 0;
 };
 
@@ -213,11 +205,11 @@ BatchLayout layout(const scigap_params* p, const scigap_batch_item* items, int n
     return q;
   };
   int total_tiles = 0;
-  size_t energy_count = 0;
-  for (int i = 0; i < n; ++i) energy_count += size_t(K) * items[i].T * D;
-  L.energy = reinterpret_cast<double*>(take(energy_count * 8));
-  L.energy_count = energy_count;
-  size_t eoff = 0;
+  size_t zc = 0;
+  for (int i = 0; i < n; ++i) zc += size_t(K) + (D >= 2 ? size_t(items[i].T) : 0);
+  L.zeroed = reinterpret_cast<int*>(take(zc * 4));
+  L.zeroed_count = int(zc);
+  size_t zoff = 0;
   L.st.resize(n);
   for (int i = 0; i < n; ++i) {
     const size_t N = size_t(items[i].H) * items[i].W, T = size_t(items[i].T);
@@ -229,10 +221,20 @@ BatchLayout layout(const scigap_params* p, const scigap_batch_item* items, int n
       for (int j = 0; j < 2; ++j) s.y1[j] = reinterpret_cast<float*>(take(N * 4));
     s.yb = reinterpret_cast<float*>(take(N * 4));
     s.stop = reinterpret_cast<int*>(take(T * 4));
-    s.nexit = reinterpret_cast<int*>(take(size_t(K) * 4));
-    s.energy = L.energy ? L.energy + eoff : nullptr;
-    eoff += size_t(K) * T * D;
-    total_tiles += batch_tiles(items[i].H, items[i].W, g, nullptr);
+    s.nexit = L.zeroed ? L.zeroed + zoff : nullptr;
+    zoff += size_t(K);
+    const int tiles = batch_tiles(items[i].H, items[i].W, g, nullptr);
+    s.eslot_stride = tiles;
+    s.eslot = reinterpret_cast<float*>(take(T * D * size_t(tiles) * 4));
+    if (D >= 2) {
+      // frames kept after D-1 updates (flags of iteration 0 are zeroed)
+      s.xalt = reinterpret_cast<float*>(take(T * N * 4));
+      s.alt[0] = L.zeroed ? L.zeroed + zoff : nullptr;
+      zoff += T;
+      s.alt[1] = reinterpret_cast<int*>(take(T * 4));
+    }
+    total_tiles += tiles;
+    L.maxT = std::max(L.maxT, int(T));
   }
   L.descs = reinterpret_cast<FusedArgs*>(take(size_t(KF > 0 ? KF : 1) * n * sizeof(FusedArgs)));
   L.fin = reinterpret_cast<FusedArgs*>(take(size_t(n) * sizeof(FusedArgs)));
@@ -316,7 +318,8 @@ int enqueue_iters(scigap_batch_plan* P, hipStream_t s) {
     k_fused_batch<D, BY, 1, MODE><<<P->grid, 64 * BY, 0, s>>>(L.descs + size_t(k) * n, L.queue,
                                                              L.qoff, L.counters + k * 8);
     SCIGAP_LAUNCH_CHECK("batched fused launch");
-    k_decide_batch<D><<<n, 256, 0, s>>>(L.descs + size_t(k) * n, L.any + k);
+    k_decide_batch<D><<<dim3(unsigned(n), unsigned(L.maxT)), 256, 0, s>>>(L.descs + size_t(k) * n,
+                                                                         L.any + k);
     SCIGAP_LAUNCH_CHECK("batched decision launch");
     k_fix_batch<D, BY, 1, MODE><<<P->grid, 64 * BY, 0, s>>>(L.descs + size_t(k) * n, L.items,
                                                             L.nitems, L.any + k);
@@ -356,7 +359,8 @@ int enqueue_all(scigap_batch_plan* P, hipStream_t s) {
   SCIGAP_LAUNCH_CHECK("counter reset");
   k_fill_i32_b<<<(KF + 255) / 256, 256, 0, s>>>(L.any, KF, 0);
   SCIGAP_LAUNCH_CHECK("counter reset");
-  if (int rc = fill_f64(L.energy, int(L.energy_count), 0.0, s)) return rc;
+  k_fill_i32_b<<<(L.zeroed_count + 255) / 256, 256, 0, s>>>(L.zeroed, L.zeroed_count, 0);
+  SCIGAP_LAUNCH_CHECK("counter reset");
   int64_t maxN = 0;
   for (auto& it : P->items) maxN = std::max<int64_t>(maxN, it.H * it.W);
   const unsigned gx = unsigned(std::min<int64_t>((maxN + 255) / 256, 1024));
@@ -426,6 +430,7 @@ int scigap_batch_plan_create(scigap_batch_plan** out, const scigap_params* p,
       a.tiles_x = tiles_x;
       a.ntiles = ntiles;
       a.per_xcd = (ntiles + 7) / 8;
+      a.eslot_tiles = ntiles;
       descs[size_t(k) * n + i] = a;
     }
     if (P->mode == ADMM) {

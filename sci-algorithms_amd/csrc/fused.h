@@ -32,12 +32,20 @@ struct FusedArgs {
   const float* truth;   // optional (T,H,W)
   double* err;          // sum of squared errors of this iteration
   float* b;             // ADMM b (T,H,W)
-  double* energy;       // [T][D] energies of this iteration
+  float* eslot;         // per-tile energy slots of this iteration, [T][D][eslot_stride]
+                        // (tv_tile.h energy_store); null: no energy test, none stored
+  int eslot_stride;     // slot row length: >= tiles of the main launch's geometry
+  int eslot_tiles;      // tiles of the main launch (slots the decision sums per row)
   float* yb;            // (H,W) sum_t Phi_t*x_t^{new} on owned pixels (for the repair)
-  int* stop;            // [T] exit iteration per frame (k_decide or the repair kernel);
+  int* stop;            // [T] exit iteration per frame (written by k_decide);
                         // null: no energy test (eps = 0), no decision/repair
-  int* nexit;           // frames of this iteration that stopped early (same writers)
-  int* stats;           // optional copy of *nexit for the caller
+  int* nexit;           // frames of this iteration that stopped early (k_decide adds;
+                        // zero before the iteration)
+  int* stats;           // optional copy of *nexit for the caller (k_decide adds)
+  float* x_alt;         // optional (T,H,W): the image after D-1 updates, written by the
+                        // main pass for the frames flagged in alt_in
+  const int* alt_in;    // [T] flags of this iteration (x_alt holds the frame), or null
+  int* alt_out;         // [T] flags for the next iteration (k_decide), or null
   int H, W, T;
   float lam, gam_next, weight, tw;
   double eps;
@@ -396,7 +404,9 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
   float* xo = a.x_out + size_t(t_lo) * N;
   const float* trt = a.truth ? a.truth + size_t(t_lo) * N : nullptr;
   float* bt = a.b ? a.b + size_t(t_lo) * N : nullptr;
-  double* en = a.energy + size_t(t_lo) * D;
+  const int es_stride = a.eslot_stride;
+  const int tile_id = tile;
+  float* es = a.eslot ? a.eslot + size_t(t_lo) * D * es_stride : nullptr;
   // Every lane reads back only the LDS slots it wrote itself (the one-barrier
   // engine synchronises before it reads a neighbour's rows).
   float2* const img = sh.img + sl0;
@@ -446,11 +456,25 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
 
     float out[RY][2];
     float acc[NH * 8];
+    // Frames flagged for this iteration (their last exit test was close; see
+    // k_decide) also keep the image after D-1 updates, so that a stop there
+    // costs the repair a load instead of a recomputation.
+    float* xa = nullptr;
+    if constexpr (D >= 2 && MODE != TV_ONLY)
+      if (a.alt_in && a.alt_in[t_lo + t]) xa = a.x_alt + size_t(t_lo + t) * N;  // uniform
+    auto save = [&](const float (&o)[RY][2]) {
+      if (xa && owns_any && tp.own_c[0]) {
+        unsigned base = unsigned(off0) * 4u;
+        compute_here(base);
+        store_owned<D, BY, MR, INSIDE>(xa, N, W, tp, base, o);
+      }
+    };
     if constexpr (X1)
-      tv_run_x1<D, RY, BY, NH, !INSIDE>(img, out, D, lds, lp, em, a.weight, a.tw, acc, true);
+      tv_run_x1<D, RY, BY, NH, !INSIDE>(img, out, D, lds, lp, em, a.weight, a.tw, acc, true,
+                                        save);
     else
       tv_run_from_zero<D, RY, BY, NH, !INSIDE>(img, out, D, lds, lp, em, a.weight, a.tw, acc,
-                                               true);
+                                               true, save);
 
     if (owns_any) {
       float bv[RY][2];
@@ -513,13 +537,12 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
     }
     const int tb = t % kEnergyFrames;
     energy_add<D, BY, MR, NH>(acc, tp.lane_owner, red + tb * BY * EnergySlots<D>::kPerWave, lp);
-    if (tb == kEnergyFrames - 1 || t == T - 1) {
-      energy_flush<D, BY>(red, tb + 1, en - size_t(tb) * D);
-    }
+    if (es && (tb == kEnergyFrames - 1 || t == T - 1))  // workgroup-uniform
+      energy_store<D, BY>(red, tb + 1, es - size_t(tb) * D * es_stride, es_stride, tile_id);
     phit += N;
     xt += N;
     xo += N;
-    en += D;
+    if (es) es += size_t(D) * es_stride;
     if constexpr (IQA) trt += N;
     if constexpr (MODE == ADMM) bt += (bt ? N : 0);
   }
@@ -598,33 +621,52 @@ __global__ void __launch_bounds__(64 * BY) k_fused(FusedArgs a) {
   }
 }
 
-// Early-exit decision: one workgroup derives every frame's exit iteration
-// from the energies of the main pass (_denoise.py:384-391) and counts the
-// frames that stopped before D updates.
+// Flag for the next iteration: keep the image after D-1 updates of a frame
+// whose exit test at D-1 passed or came within kAltMargin times its threshold.
+constexpr double kAltMargin = 4.0;
+
+// Early-exit decision of one frame (workgroup blockIdx.x; 256 threads): the
+// energies E_0..E_{D-1} are the sums of the frame's per-tile slots, taken in
+// a fixed order (slot_row_sum), then the exit test of _denoise.py:384-391.
+// Writes stop[t], adds early stops to *nexit and *stats, and the x_alt flag
+// of the frame for the next iteration.
 template <int D>
-__global__ void __launch_bounds__(256) k_decide(const double* __restrict__ energy, int T,
-                                                double eps, int* stop, int* nexit, int* stats) {
-  __shared__ int cnt;
-  if (threadIdx.x == 0) cnt = 0;
-  __syncthreads();
-  for (int t = threadIdx.x; t < T; t += blockDim.x) {
-    const int s = exit_iteration<D>(energy + size_t(t) * D, eps);
-    stop[t] = s;
-    if (s < D) atomicAdd(&cnt, 1);
+__device__ __forceinline__ void decide_frame(const FusedArgs& a, int t) {
+  __shared__ double e[D];
+  const int w = threadIdx.x >> 6;
+  for (int u = w; u < D; u += 4) {  // wave-uniform
+    const double v = slot_row_sum(a.eslot + (size_t(t) * D + u) * a.eslot_stride, a.eslot_tiles);
+    if ((threadIdx.x & 63) == 0) e[u] = v;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    *nexit = cnt;
-    if (stats) *stats = cnt;
+    const int st = exit_iteration<D>(e, a.eps);
+    a.stop[t] = st;
+    if (st < D) {
+      atomicAdd(a.nexit, 1);
+      if (a.stats) atomicAdd(a.stats, 1);
+    }
+    if (a.alt_out) {
+      int flag = 0;
+      if constexpr (D >= 2) flag = fabs(e[D - 2] - e[D - 1]) < kAltMargin * a.eps * e[0];
+      a.alt_out[t] = flag;
+    }
   }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_decide(FusedArgs a) {
+  decide_frame<D>(a, blockIdx.x);
 }
 
 // Early-exit repair (does work only when some frame stopped early; `stop`
 // holds each frame's exit iteration, D for none).
-// For every frame that stopped at s < D, the frame is recomputed with s
-// updates from the unchanged inputs (x_in and r_in are double-buffered), its
-// contribution to yb is corrected by Phi*(x_new - x_old) (ADMM: 2*Phi*...),
-// and the residual epilogue is recomputed from the corrected yb.
+// For every frame that stopped at s < D, the frame's image after s updates
+// replaces the main pass's: loaded from x_alt when s = D-1 and the main pass
+// kept it, else recomputed with s updates from the unchanged inputs (x_in
+// and r_in are double-buffered).  Its contribution to yb is corrected by
+// Phi*(x_new - x_old) (ADMM: 2*Phi*...) and the residual epilogue is
+// recomputed from the corrected yb.
 template <int D, int BY, int MR>
 struct FixLds {
   TileLds<BY> eng;
@@ -671,10 +713,21 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
   float dyb[RY][2] = {};
   float err = 0.f;
 
+  // A stopped frame's image comes from x_alt when it stopped after D-1
+  // updates and the main pass kept that image (wave-uniform per frame).
+  auto from_alt = [&](int t) {
+    if constexpr (kTvOnly || D < 2) return false;
+    return stop[t] == D - 1 && a.alt_in && a.alt_in[t] != 0;
+  };
+  // The TV input img = x + q*Phi is needed for a recomputation and for
+  // ADMM's b = x_new - v.
+  auto needs_x = [&](int t) {
+    return !from_y && (!from_alt(t) || (MODE == ADMM && a.write_b));
+  };
   // The stopped frames are visited in order.  A frame's x and Phi are
   // requested one stopped frame ahead and its old output before its TV, so
   // that their latency hides behind the stencil arithmetic (as in the main
-  // pass); nx stays 0 when X_0 = At(y).
+  // pass); nx stays 0 when X_0 = At(y) or when x is not needed.
   auto next_stopped = [&](int t) {
     while (t < T && stop[t] >= D) ++t;
     return t;
@@ -683,12 +736,13 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
   float nph[RY][2] = {}, nx[RY][2] = {};
   if (t < T) {
     if constexpr (!kTvOnly) load_plane<RY, INSIDE>(a.phi + size_t(t) * N, nph, lp, off0, H, W);
-    if (!from_y) load_plane<RY, INSIDE>(a.x_in + size_t(t) * N, nx, lp, off0, H, W);
+    if (needs_x(t)) load_plane<RY, INSIDE>(a.x_in + size_t(t) * N, nx, lp, off0, H, W);
   }
   // Every lane reads back only the LDS slots it wrote itself.
   float2* const simg = sh.img + lp.w * RY * 64 + lp.lane;
   while (t < T) {
     const int nup = stop[t];
+    const bool alt = from_alt(t);
     float ph[RY][2];
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
@@ -701,15 +755,20 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
     float* xo = a.x_out + size_t(t) * N;
     float xold[RY][2] = {};
     if constexpr (!kTvOnly) load_plane<RY, INSIDE>(xo, xold, lp, off0, H, W);
+    float out[RY][2];
+    if (alt) load_plane<RY, INSIDE>(a.x_alt + size_t(t) * N, out, lp, off0, H, W);
     const int tn = next_stopped(t + 1);
     if (tn < T) {
+#pragma unroll
+      for (int j = 0; j < RY; ++j) nx[j][0] = nx[j][1] = 0.f;
       if constexpr (!kTvOnly) load_plane<RY, INSIDE>(a.phi + size_t(tn) * N, nph, lp, off0, H, W);
-      if (!from_y) load_plane<RY, INSIDE>(a.x_in + size_t(tn) * N, nx, lp, off0, H, W);
+      if (needs_x(tn)) load_plane<RY, INSIDE>(a.x_in + size_t(tn) * N, nx, lp, off0, H, W);
     }
-    float out[RY][2];
-    float acc[NH * 8];
-    tv_run_from_zero<D, RY, BY, NH, !INSIDE>(simg, out, nup, lds, lp, em, a.weight, a.tw, acc,
-                                             false);
+    if (!alt) {
+      float acc[NH * 8];
+      tv_run_from_zero<D, RY, BY, NH, !INSIDE>(simg, out, nup, lds, lp, em, a.weight, a.tw, acc,
+                                               false);
+    }
     const float* trt = a.truth ? a.truth + size_t(t) * N : nullptr;
     float* bt = a.b ? a.b + size_t(t) * N : nullptr;
 #pragma unroll
@@ -755,50 +814,21 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
     if (a.truth && a.err) commit_err<BY>(err, a.err, red64, lp.w, lp.lane);
 }
 
-// Solves of up to kDecideFrames frames need no separate decision launch:
-// every repair workgroup derives the exit iterations itself from the T*D
-// energies (into LDS), and workgroup 0 publishes them (stop, nexit, stats)
-// as k_decide would.  Longer solves run k_decide first.
-constexpr int kDecideFrames = 1024;
-
 // For D <= 4 at least 4 waves per SIMD: two 8-wave workgroups per CU
-// (128 VGPRs); deeper tiles need more registers than that.
+// (128 VGPRs); deeper tiles need more registers than that.  Runs after
+// k_decide; returns at once when no frame stopped early (the common case).
 template <int D, int BY, int MR, int MODE>
 __global__ void __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(D <= 4 ? 4 : 1)))
 k_fused_fix(FusedArgs a) {
   __shared__ FixLds<D, BY, MR> sh;
-  __shared__ int s_stop[kDecideFrames];
-  __shared__ int s_count;
-  const int* stop = a.stop;
-  if (a.T <= kDecideFrames) {
-    if (threadIdx.x == 0) s_count = 0;
-    __syncthreads();
-    int mine = 0;
-    for (int t = threadIdx.x; t < a.T; t += blockDim.x) {
-      const int st = exit_iteration<D>(a.energy + size_t(t) * D, a.eps);
-      s_stop[t] = st;
-      if (blockIdx.x == 0) a.stop[t] = st;
-      mine += st < D;
-    }
-    if (mine) atomicAdd(&s_count, mine);
-    __syncthreads();
-    const int count = s_count;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      *a.nexit = count;
-      if (a.stats) *a.stats = count;
-    }
-    if (count == 0) return;  // the common case
-    stop = s_stop;
-  } else if (*a.nexit == 0) {
-    return;  // decided by k_decide: one scalar load per workgroup
-  }
+  if (*a.nexit == 0) return;  // one scalar load per workgroup
   const int tile = tile_of_block(blockIdx.x, a.per_xcd);
   if (tile >= a.ntiles) return;
   // Interior tiles run without masks or bounds checks (as in the main pass).
   if (tile_inside<D, BY, MR>(tile, a.tiles_x, a.H, a.W))
-    fused_fix_tile<D, BY, MR, MODE, true>(a, tile, sh, stop);
+    fused_fix_tile<D, BY, MR, MODE, true>(a, tile, sh, a.stop);
   else
-    fused_fix_tile<D, BY, MR, MODE, false>(a, tile, sh, stop);
+    fused_fix_tile<D, BY, MR, MODE, false>(a, tile, sh, a.stop);
 }
 
 // Initial residual: X_0 = x0 (or At(y)), yb = A(X_0), then
@@ -993,7 +1023,10 @@ struct SolveState {
   float* yb_part = nullptr;  // [parts][H*W] partial yb of frame-split launches
   int* stop = nullptr;       // [T]
   int* nexit = nullptr;      // [iters]
-  double* energy = nullptr;  // [iters][T][D]
+  float* eslot = nullptr;    // [T][D][eslot_stride] per-tile energies (reused every iteration)
+  int eslot_stride = 0;
+  float* xalt = nullptr;     // (T,H,W) image after D-1 updates of flagged frames, or null
+  int* alt[2] = {nullptr, nullptr};  // [T] flags, ping-pong between iterations
 };
 
 inline int solver_mode(const scigap_params& p) {
@@ -1037,7 +1070,11 @@ inline FusedArgs iteration_args(const scigap_params& p, int k, int64_t H, int64_
   a.truth = truth;
   a.err = truth ? sq_err + k : nullptr;
   a.b = st.bbuf;
-  a.energy = st.energy + size_t(k) * T * (D > 0 ? D : 1);
+  a.eslot = st.eslot;
+  a.eslot_stride = st.eslot_stride;
+  a.x_alt = st.xalt;
+  a.alt_in = st.xalt ? st.alt[k & 1] : nullptr;
+  a.alt_out = st.xalt ? st.alt[(k + 1) & 1] : nullptr;
   a.yb = st.yb;
   a.yb_part = st.yb_part;
   a.stop = st.stop;

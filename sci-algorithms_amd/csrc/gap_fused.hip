@@ -202,9 +202,32 @@ int launch_mode(const FusedArgs& a, hipStream_t s, int engine) {
   return launch_main<D, BY, TV_ONLY, false>(a, s, engine);
 }
 
+// Tiles of the main launch for (D, BY): the energy slots' row length.
+template <int D, int BY>
+int main_tiles(int H, int W) {
+  using G = TileGeom<D, BY, 1>;
+  return tiles_x_of<D, BY, 1>(W) * ((H + G::OY - 1) / G::OY);
+}
+
 template <int D>
-int launch_fused(const FusedArgs& a0, hipStream_t s, int tile_rows, int engine) {
+int main_tiles_for(int H, int W, int tile_rows) {
+  if constexpr (D == 4)
+    return tile_rows == 12 ? main_tiles<4, 12>(H, W)
+           : tile_rows == 16 ? main_tiles<4, 16>(H, W) : main_tiles<4, 8>(H, W);
+  else
+    return main_tiles<D, FusedShape<D>::BY>(H, W);
+}
+
+template <int D>
+int launch_fused(const FusedArgs& a00, hipStream_t s, int tile_rows, int engine) {
   constexpr int BY = FusedShape<D>::BY, MR = FusedShape<D>::MR;
+  FusedArgs a0 = a00;
+  if (a0.eslot) {
+    a0.eslot_tiles = main_tiles_for<D>(a0.H, a0.W, tile_rows);
+    if (a0.eslot_tiles > a0.eslot_stride)
+      return fail(SCIGAP_EWORKSPACE, "energy slots hold %d tiles, the launch has %d",
+                  a0.eslot_stride, a0.eslot_tiles);
+  }
   int rc;
   if constexpr (D == 4) {
     // 8 waves measured fastest at 2048x2048x20 (419 us against 509 us for
@@ -220,12 +243,13 @@ int launch_fused(const FusedArgs& a0, hipStream_t s, int tile_rows, int engine) 
   // No stop array: the caller runs without the energy test (eps = 0), so no
   // frame can stop early and there is nothing to decide or repair.
   if (!a0.stop) return SCIGAP_OK;
+  if (!a0.eslot) return fail(SCIGAP_EINVAL, "early exit without energy slots");
   FusedArgs a = a0;
   constexpr int FBY = FixShape<D>::BY;
   const unsigned grid = set_tiles<D, FBY, MR>(a);
-  if (a.T > kDecideFrames) {  // otherwise the repair kernel decides
+  {
     ProfScope ps("fused_decide", s);
-    k_decide<D><<<1, 256, 0, s>>>(a.energy, a.T, a.eps, a.stop, a.nexit, a.stats);
+    k_decide<D><<<unsigned(a.T), 256, 0, s>>>(a);
     SCIGAP_LAUNCH_CHECK("early-exit decision launch");
   }
   {
@@ -252,6 +276,22 @@ int launch_fused_depth(int D, const FusedArgs& a, hipStream_t s, int tile_rows =
     case 6: return launch_fused<6>(a, s, tile_rows, engine);
   }
   return fail(SCIGAP_EUNSUPPORTED, "fused depth %d not instantiated", D);
+}
+
+// Longest energy-slot row any main launch of depth D can need at HxW (the
+// tile heights selectable for D = 4 included).
+int max_slot_tiles(int D, int H, int W) {
+  switch (D) {
+    case 1: return main_tiles<1, FusedShape<1>::BY>(H, W);
+    case 2: return main_tiles<2, FusedShape<2>::BY>(H, W);
+    case 3: return main_tiles<3, FusedShape<3>::BY>(H, W);
+    case 4:
+      return std::max(main_tiles<4, 8>(H, W),
+                      std::max(main_tiles<4, 12>(H, W), main_tiles<4, 16>(H, W)));
+    case 5: return main_tiles<5, FusedShape<5>::BY>(H, W);
+    case 6: return main_tiles<6, FusedShape<6>::BY>(H, W);
+  }
+  return 0;
 }
 
 unsigned grid_pixels(int64_t N) {
@@ -283,7 +323,14 @@ Workspace carve(const scigap_params* p, char* base) {
   for (int i = 0; i < 2; ++i) w.st.r[i] = reinterpret_cast<float*>(take(N * 4));
   if (p->method == SCIGAP_METHOD_GAP && p->accelerate)
     for (int i = 0; i < 2; ++i) w.st.y1[i] = reinterpret_cast<float*>(take(N * 4));
-  w.st.energy = reinterpret_cast<double*>(take(K * T * D * 8));
+  if (p->tv_iter_max - 1 >= 1 && p->tv_iter_max - 1 <= kMaxFused) {
+    w.st.eslot_stride = max_slot_tiles(D, int(p->H), int(p->W));
+    w.st.eslot = reinterpret_cast<float*>(take(T * D * size_t(w.st.eslot_stride) * 4));
+    if (D >= 2) {
+      w.st.xalt = reinterpret_cast<float*>(take(T * N * 4));
+      for (int i = 0; i < 2; ++i) w.st.alt[i] = reinterpret_cast<int*>(take(T * 4));
+    }
+  }
   w.st.yb = reinterpret_cast<float*>(take(N * 4));
   w.st.yb_part = reinterpret_cast<float*>(take(std::min<size_t>(T, kMaxParts) * N * 4));
   w.st.stop = reinterpret_cast<int*>(take(T * 4));
@@ -325,16 +372,14 @@ int check_params(const scigap_params* p) {
 // |E_prev - E| < eps*E_0).  xs holds x_0 on entry; iteration k reads x_k from
 // xs + k*xstride and writes x_{k+1} to xs + (k+1)*xstride (the last one to
 // x_out); the residual r_k it uses is written to rs + k*rstride (strides in
-// floats).  y1 ping-pongs between the two planes of y1b; energy holds K*T*D
-// doubles, nexit K ints.
+// floats).  y1 ping-pongs between the two planes of y1b; nexit holds K ints.
 int unrolled_forward_iterations(int64_t H, int64_t W, int64_t T, int K, int D,
                                 const float* lambdas, const float* weights, const float* y,
                                 const float* phi, const float* S, float* xs, size_t xstride,
                                 float* y1b, float* x_out, float* rs, size_t rstride,
-                                double* energy, int* nexit, float* yb_part, hipStream_t s) {
+                                int* nexit, float* yb_part, hipStream_t s) {
   const int64_t N = H * W;
   if (D < 1 || D > kMaxFused) return fail(SCIGAP_EUNSUPPORTED, "unrolled depth %d", D);
-  if (int rc = fill_f64(energy, K * int(T) * D, 0.0, s)) return rc;
   {
     ProfScope ps("init", s);
     k_init<<<grid_pixels(N), 256, 0, s>>>(xs, phi, y, S, rs, y1b, N, int(T), GAP_ACC, 0.f);
@@ -351,8 +396,8 @@ int unrolled_forward_iterations(int64_t H, int64_t W, int64_t T, int K, int D,
     a.r_out = k + 1 < K ? rs + size_t(k + 1) * rstride : nullptr;
     a.y1_in = y1b + size_t(k & 1) * N;
     a.y1_out = k + 1 < K ? y1b + size_t((k + 1) & 1) * N : nullptr;
-    a.energy = energy + size_t(k) * T * D;
-    a.stop = nullptr;  // eps = 0: no early exit, no decision or repair pass
+    a.eslot = nullptr;  // eps = 0: no early exit, no energies, decision or repair pass
+    a.stop = nullptr;
     a.nexit = nexit + k;
     a.yb_part = yb_part;
     a.H = int(H);
@@ -438,8 +483,15 @@ int scigap_solve(const scigap_params* p, const float* y, const float* phi, const
     }
     return scigap_adjoint(y, phi, x_out, p->H, p->W, T, stream);
   }
-  if (!deep && D > 0)
-    if (int rc = fill_f64(ws.st.energy, K * int(T) * D, 0.0, s)) return rc;
+  if (!deep && D > 0) {
+    // early-stop counters of every iteration; no frame kept for iteration 0
+    k_fill_i32<<<unsigned((K + 255) / 256), 256, 0, s>>>(ws.st.nexit, K, 0);
+    SCIGAP_LAUNCH_CHECK("counter fill");
+    if (ws.st.alt[0]) {
+      k_fill_i32<<<unsigned((T + 255) / 256), 256, 0, s>>>(ws.st.alt[0], int(T), 0);
+      SCIGAP_LAUNCH_CHECK("flag fill");
+    }
+  }
   if (truth)
     if (int rc = fill_f64(sq_err, K, 0.0, s)) return rc;
   if (stats) {
@@ -497,7 +549,9 @@ size_t scigap_tv_workspace_bytes_depth(int64_t H, int64_t W, int64_t T, int n_it
   const int D = n_iter_max - 1;
   if (D == 0) return 256;
   if (depth == 0 && D <= kMaxFused) depth = D;
-  if (depth >= D) return align256(size_t(T) * D * 8) + align256(size_t(T) * 4) + 256;
+  if (depth >= D)
+    return align256(size_t(T) * D * size_t(max_slot_tiles(D, int(H), int(W))) * 4) +
+           align256(size_t(T) * 4) + 256;
   return tv_chunked_workspace(H, W, T, n_iter_max, D);
 }
 
@@ -542,15 +596,18 @@ int scigap_tv_chambolle_depth(const float* image, float* out, int64_t H, int64_t
     return tv_chunked(image, out, H, W, T, weight, eps, n_iter_max, depth, stop_iter, workspace,
                       s);
   // All updates in one launch from p = 0.
-  double* energy = static_cast<double*>(workspace);
-  char* rest = static_cast<char*>(workspace) + align256(size_t(T) * D * 8);
-  if (int rc = fill_f64(energy, int(T) * D, 0.0, s)) return rc;
+  const int stride = max_slot_tiles(D, int(H), int(W));
+  float* slots = static_cast<float*>(workspace);
+  char* rest = static_cast<char*>(workspace) + align256(size_t(T) * D * size_t(stride) * 4);
   FusedArgs a{};
   a.x_in = image;
   a.x_out = out;
-  a.energy = energy;
+  a.eslot = slots;
+  a.eslot_stride = stride;
   a.stop = stop_iter ? stop_iter : reinterpret_cast<int*>(rest);
   a.nexit = reinterpret_cast<int*>(rest + align256(size_t(T) * 4));
+  k_fill_i32<<<1, 64, 0, s>>>(a.nexit, 1, 0);
+  SCIGAP_LAUNCH_CHECK("counter fill");
   a.H = int(H);
   a.W = int(W);
   a.T = int(T);

@@ -239,7 +239,7 @@ int unrolled_forward_iterations(int64_t H, int64_t W, int64_t T, int K, int D,
                                 const float* lambdas, const float* weights, const float* y,
                                 const float* phi, const float* S, float* xs, size_t xstride,
                                 float* y1b, float* x_out, float* rs, size_t rstride,
-                                double* energy, int* nexit, float* yb_part, hipStream_t s);
+                                int* nexit, float* yb_part, hipStream_t s);
 
 namespace {
 
@@ -268,7 +268,6 @@ constexpr int kLanes = 4;
 
 struct ULane {
   float* y1b = nullptr;     // two y1 planes (forward)
-  double* energy = nullptr; // K*T*D
   int* nexit = nullptr;     // K
   float* yb_part = nullptr; // [min(T,8)][H*W] partial yb of frame-split launches
   float* ybar = nullptr;    // (H,W) dL/dy of one sample
@@ -308,7 +307,6 @@ UWorkspace carve(const scigap_unrolled& u, char* base) {
   for (int l = 0; l < w.lanes; ++l) {
     ULane& L = w.lane[l];
     L.y1b = reinterpret_cast<float*>(take(2 * N * 4));
-    L.energy = reinterpret_cast<double*>(take(K * size_t(u.T) * size_t(D) * 8));
     L.nexit = reinterpret_cast<int*>(take(K * 4));
     L.yb_part = reinterpret_cast<float*>(take(size_t(u.T < 8 ? u.T : 8) * N * 4));
     L.ybar = reinterpret_cast<float*>(take(N * 4));
@@ -494,7 +492,7 @@ int scigap_unrolled_forward(const scigap_unrolled* u, const float* lambdas, cons
     if (int rc = unrolled_forward_iterations(u->H, u->W, u->T, u->iters, u->tv_iter_max - 1,
                                              lambdas, weights, yb, phi, w.S, xs, cl.xstride,
                                              L.y1b, x_out + size_t(b) * V, rs, cl.rstride,
-                                             L.energy, L.nexit, L.yb_part, ls))
+                                             L.nexit, L.yb_part, ls))
       return rc;
   }
   return lanes.join();

@@ -135,13 +135,15 @@ __global__ void __launch_bounds__(kThreads) k_sq_error(const float* __restrict__
                                                        double* __restrict__ out, int64_t n,
                                                        int per_slice_out) {
   const int64_t base = int64_t(blockIdx.y) * n;
-  float acc = 0.f;
+  // As scikit-image's mean_squared_error for float32 images: the difference
+  // is squared in fp32 and the squares are accumulated in fp64.
+  double acc = 0.0;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
-    float d = a[base + i] - b[base + i];
-    acc = fmaf(d, d, acc);
+    const float d = a[base + i] - b[base + i];
+    acc += double(d * d);
   }
-  double s = wave_sum(double(acc));
+  double s = wave_sum(acc);
   __shared__ double part[kThreads / 64];
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
   __syncthreads();

@@ -11,8 +11,9 @@
 //   out_{i0} = img + d(p_{i0}), runs nup updates with the energies
 //   E_{i0} .. E_{i0+nup-1}, and stores p_{i0+nup} (p is double-buffered
 //   across chunks) -- and out on the last chunk;
-//   a one-workgroup decision kernel then applies the exit test of
-//   _denoise.py:384-391 to those energies for every running frame;
+//   a decision kernel (one workgroup per frame) sums the chunk's per-tile
+//   energy slots in a fixed order, stores E_{i0} .. E_{i0+nup-1} in fp64 and
+//   applies the exit test of _denoise.py:384-391 for every running frame;
 //   a repair kernel (a no-op unless a frame stopped) recomputes, for each
 //   frame that stopped at s inside the chunk, out_s from p_{i0} with s - i0
 //   updates.  Stopped frames are skipped by later chunks.
@@ -24,9 +25,9 @@
 
 #include "fused.h"
 
-namespace scigap {
+#include <algorithm>
 
-int fill_f64(double* p, int n, double v, hipStream_t s);
+namespace scigap {
 
 namespace {
 
@@ -39,7 +40,9 @@ struct ChunkArgs {
   const float* p_in;  // [T][2][N] p_{i0}, or null for i0 = 0
   float* p_out;       // [T][2][N] p_{i0+nup}, or null on the last chunk
   float* out;         // (T,H,W)
-  double* energy;     // [T][ne] E_0 .. E_{ne-1}
+  double* energy;     // [T][ne] E_0 .. E_{ne-1} (written by the decision kernels)
+  float* eslot;       // [T][D][eslot_stride] per-tile energies of this chunk
+  int eslot_stride;   // tiles of the chunk geometry
   int* stop;          // [T] iteration whose image is returned, -1 while running
   int* fix;           // [T] updates to redo for frames that stopped in this chunk, else -1
   int* nfix;          // frames that stopped in this chunk
@@ -147,17 +150,11 @@ __device__ __forceinline__ void chunk_tile(const ChunkArgs& a, int tile, ChunkLd
     if constexpr (!REPAIR) {
       const int tb = t % kEnergyFrames;
       if (tb == kEnergyFrames - 1 || t == T - 1) {
-        // Strided flush into E[t][i0 + u], u < nup.
-        // Slots of frames skipped in this block keep older values; those
-        // frames have stopped and their energies are never read again.
-        __syncthreads();
+        // Slots of frames skipped in this block hold stale values; those
+        // frames have stopped and the decision never reads them.
         const int t0 = t - tb;
-        for (int i = threadIdx.x; i < (tb + 1) * D; i += blockDim.x) {
-          const int tl = i / D, u = i - tl * D;
-          const float sum = energy_block_sum<D, BY>(sh.red, tl, u);
-          if (u < a.nup) atomicAdd(a.energy + size_t(t0 + tl) * a.ne + a.i0 + u, double(sum));
-        }
-        __syncthreads();
+        energy_store<D, BY>(sh.red, tb + 1, a.eslot + size_t(t0) * D * a.eslot_stride,
+                            a.eslot_stride, tile);
       }
     }
   }
@@ -183,28 +180,37 @@ __global__ void __launch_bounds__(64 * BY) k_tv_chunk_fix(ChunkArgs a) {
   chunk_tile<D, BY, false, true>(a, tile, sh);
 }
 
-// Exit test (_denoise.py:384-391) over the iterations of this chunk.
+// Energies of this chunk and the exit test (_denoise.py:384-391) over its
+// iterations, one workgroup per frame (grid T; D is the chunk geometry's
+// depth, slot rows u < nup are this chunk's updates).
+template <int D>
 __global__ void __launch_bounds__(256) k_chunk_decide(ChunkArgs a) {
-  __shared__ int cnt;
-  if (threadIdx.x == 0) cnt = 0;
+  const int t = blockIdx.x;
+  if (a.stop[t] >= 0) {  // stopped in an earlier chunk (workgroup-uniform)
+    if (threadIdx.x == 0) a.fix[t] = -1;
+    return;
+  }
+  double* e = a.energy + size_t(t) * a.ne;
+  const int w = threadIdx.x >> 6;
+  for (int u = w; u < a.nup; u += 4) {  // wave-uniform
+    const double v = slot_row_sum(a.eslot + (size_t(t) * D + u) * a.eslot_stride,
+                                  a.eslot_stride);
+    if ((threadIdx.x & 63) == 0) e[a.i0 + u] = v;
+  }
+  __threadfence_block();
   __syncthreads();
-  for (int t = threadIdx.x; t < a.T; t += blockDim.x) {
+  if (threadIdx.x == 0) {
     int f = -1;
-    if (a.stop[t] < 0) {
-      const double* e = a.energy + size_t(t) * a.ne;
-      const double e0 = e[0];
-      for (int i = a.i0 > 1 ? a.i0 : 1; i < a.i0 + a.nup; ++i)
-        if (fabs(e[i - 1] - e[i]) < a.eps * e0) {
-          a.stop[t] = i;
-          f = i - a.i0;
-          atomicAdd(&cnt, 1);
-          break;
-        }
-    }
+    const double e0 = e[0];
+    for (int i = a.i0 > 1 ? a.i0 : 1; i < a.i0 + a.nup; ++i)
+      if (fabs(e[i - 1] - e[i]) < a.eps * e0) {
+        a.stop[t] = i;
+        f = i - a.i0;
+        atomicAdd(a.nfix, 1);
+        break;
+      }
     a.fix[t] = f;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) *a.nfix = cnt;
 }
 
 __global__ void k_stop_final(const int* stop, int* stop_iter, int T, int ne) {
@@ -215,6 +221,13 @@ __global__ void k_stop_final(const int* stop, int* stop_iter, int T, int ne) {
 __global__ void k_fill_int(int* p, int n, int v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+template <int D>
+int chunk_tiles(int H, int W) {
+  constexpr int BY = ChunkShape<D>::BY;
+  using G = TileGeom<D, BY, 1>;
+  return tiles_x_of<D, BY, 1>(W) * ((H + G::OY - 1) / G::OY);
 }
 
 // Chunks of up to d updates run on the tile geometry of depth D = d + 1:
@@ -228,6 +241,7 @@ int launch_chunk(ChunkArgs a, hipStream_t s) {
   a.tiles_x = tiles_x_of<D, BY, 1>(a.W);
   a.ntiles = a.tiles_x * ((a.H + G::OY - 1) / G::OY);
   a.per_xcd = (a.ntiles + 7) / 8;
+  a.eslot_stride = a.ntiles;
   const unsigned grid = unsigned(a.per_xcd * 8);
   {
     ProfScope ps("tv_chunk", s);
@@ -236,7 +250,7 @@ int launch_chunk(ChunkArgs a, hipStream_t s) {
   SCIGAP_LAUNCH_CHECK("tv chunk launch");
   {
     ProfScope ps("tv_chunk_decide", s);
-    k_chunk_decide<<<1, 256, 0, s>>>(a);
+    k_chunk_decide<D><<<unsigned(a.T), 256, 0, s>>>(a);
   }
   SCIGAP_LAUNCH_CHECK("tv chunk decision launch");
   {
@@ -263,11 +277,25 @@ int launch_chunk_depth(int d, const ChunkArgs& a, hipStream_t s) {
 
 int max_chunk_depth() { return kMaxChunk; }
 
+// Energy slots of one chunk: the most tiles any chunk geometry has at HxW,
+// times (T frames x 7 slot rows, the deepest chunk geometry).
+size_t chunk_slot_floats(int64_t H, int64_t W, int64_t T) {
+  int most = 0;
+  most = std::max(most, chunk_tiles<2>(int(H), int(W)));
+  most = std::max(most, chunk_tiles<3>(int(H), int(W)));
+  most = std::max(most, chunk_tiles<4>(int(H), int(W)));
+  most = std::max(most, chunk_tiles<5>(int(H), int(W)));
+  most = std::max(most, chunk_tiles<6>(int(H), int(W)));
+  most = std::max(most, chunk_tiles<7>(int(H), int(W)));
+  return size_t(T) * (kMaxChunk + 1) * size_t(most);
+}
+
 size_t tv_chunked_workspace(int64_t H, int64_t W, int64_t T, int n, int nchunks_max) {
   const size_t N = size_t(H) * W;
   const int ne = n > 1 ? n - 1 : 1;
   return 2 * align256(2 * size_t(T) * N * 4) + align256(size_t(T) * ne * 8) +
-         2 * align256(size_t(T) * 4) + align256(size_t(nchunks_max > 0 ? nchunks_max : 1) * 4);
+         align256(chunk_slot_floats(H, W, T) * 4) + 2 * align256(size_t(T) * 4) +
+         align256(size_t(nchunks_max > 0 ? nchunks_max : 1) * 4);
 }
 
 // TV of `image` (T,H,W) into `out` with n_iter_max = n >= 2, in chunks of
@@ -289,16 +317,19 @@ int tv_chunked(const float* image, float* out, int64_t H, int64_t W, int64_t T, 
   P[0] = reinterpret_cast<float*>(take(2 * size_t(T) * N * 4));
   P[1] = reinterpret_cast<float*>(take(2 * size_t(T) * N * 4));
   double* energy = reinterpret_cast<double*>(take(size_t(T) * ne * 8));
+  float* slots = reinterpret_cast<float*>(take(chunk_slot_floats(H, W, T) * 4));
   int* stop = reinterpret_cast<int*>(take(size_t(T) * 4));
   int* fix = reinterpret_cast<int*>(take(size_t(T) * 4));
   int* nfix = reinterpret_cast<int*>(take(size_t(nchunks) * 4));
-  if (int rc = fill_f64(energy, int(T) * ne, 0.0, s)) return rc;
   k_fill_int<<<unsigned((T + 255) / 256), 256, 0, s>>>(stop, int(T), -1);
   SCIGAP_LAUNCH_CHECK("stop fill");
+  k_fill_int<<<unsigned((nchunks + 255) / 256), 256, 0, s>>>(nfix, nchunks, 0);
+  SCIGAP_LAUNCH_CHECK("counter fill");
   ChunkArgs a{};
   a.img = image;
   a.out = out;
   a.energy = energy;
+  a.eslot = slots;
   a.stop = stop;
   a.fix = fix;
   a.H = int(H);

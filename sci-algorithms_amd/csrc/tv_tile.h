@@ -186,17 +186,26 @@ __device__ __forceinline__ void tv_grad(const float (&out)[RY][2], float (&p0)[R
     }
 }
 
+// Callback of the engines, handed the image after D-1 updates (before the
+// last one); the default does nothing.
+struct NoSave {
+  template <class V>
+  __device__ void operator()(const V&) const {}
+};
+
 // Run `nup` (<= D) updates starting from p = 0 (out = img).  `img` points at
 // this lane's float2 of register row 0 in an LDS plane laid out [row][64 lanes]
 // (row j at img[j*64]).  On return `out` holds the image after nup updates and
 // acc.e[h][u] this lane's partial of E_u over half h of its strip.  When
 // `accumulate` is false the energies are not formed (recomputation passes).
-// Energy slots are addressed as e[h][u] = base[h*8 + u], so D <= 8.
-template <int D, int RY, int BY, int NH, bool EDGE>
+// Energy slots are addressed as e[h][u] = base[h*8 + u], so D <= 8.  `save`
+// receives `out` after D-1 updates when nup = D.
+template <int D, int RY, int BY, int NH, bool EDGE, class Save = NoSave>
 __device__ __forceinline__ void tv_run_from_zero(const float2* img, float (&out)[RY][2], int nup,
                                                  TileLds<BY>& lds, const LanePos& lp,
                                                  const EdgeMasks& em, float weight, float tw,
-                                                 float (&acc)[NH * 8], bool accumulate) {
+                                                 float (&acc)[NH * 8], bool accumulate,
+                                                 const Save& save = Save()) {
   float p0[RY][2], p1[RY][2];
 #pragma unroll
   for (int j = 0; j < RY; ++j) {
@@ -214,6 +223,7 @@ __device__ __forceinline__ void tv_run_from_zero(const float2* img, float (&out)
 #pragma unroll
   for (int u = 0; u < D; ++u) {
     if (u < nup) {
+      if (u == D - 1) save(out);
       tv_grad<RY, BY, NH, EDGE>(out, p0, p1, lds, lp, em, weight, tw, accumulate ? &acc[u] : nullptr);
       const bool dn = accumulate && u + 1 < D;
       tv_div<RY, BY, NH>(img, out, p0, p1, lds, lp, dn ? &acc[u + 1] : nullptr);
@@ -311,11 +321,12 @@ __device__ __forceinline__ void tv_div_regs(const float2* img, float (&out)[RY][
 // tv_run_from_zero with one barrier per update (plus one at entry, after
 // which the img rows of the wave below are read).  Same arguments and
 // results, bit for bit.
-template <int D, int RY, int BY, int NH, bool EDGE>
+template <int D, int RY, int BY, int NH, bool EDGE, class Save = NoSave>
 __device__ __forceinline__ void tv_run_x1(const float2* img, float (&out)[RY][2], int nup,
                                           TileLdsX1<BY>& lds, const LanePos& lp,
                                           const EdgeMasks& em, float weight, float tw,
-                                          float (&acc)[NH * 8], bool accumulate) {
+                                          float (&acc)[NH * 8], bool accumulate,
+                                          const Save& save = Save()) {
   float p0[RY][2], p1[RY][2];
 #pragma unroll
   for (int j = 0; j < RY; ++j) {
@@ -339,6 +350,7 @@ __device__ __forceinline__ void tv_run_x1(const float2* img, float (&out)[RY][2]
 #pragma unroll
   for (int u = 0; u < D; ++u) {
     if (u < nup) {
+      if (u == D - 1) save(out);
       tv_grad_regs<RY, NH, EDGE>(out, below, p0, p1, em, weight, tw,
                                  accumulate ? &acc[u] : nullptr);
       const int par = u & 1;
@@ -366,13 +378,16 @@ __device__ __forceinline__ void tv_run_x1(const float2* img, float (&out)[RY][2]
 
 // Frame energies.  Each wave reduces its lanes' partials and writes the
 // result to its own LDS slots of the frame ([frame % kEnergyFrames][wave]
-// [slot] floats), so frames need no barrier and no atomics; energy_flush sums
-// the waves of a block of frames in a fixed order and adds the totals to the
-// global fp64 accumulators (one barrier pair and T*D atomics per workgroup
-// per block).  Halo waves hold no interior rows and never write: their slots
-// keep the zeros written at the start of the tile.  A lane contributes half h
-// of its strip when those rows are interior rows of the tile and its columns
-// are interior columns.
+// [slot] floats), so frames need no barrier and no atomics; energy_store sums
+// the waves of a block of frames in a fixed order and writes one total per
+// (frame, update) to the tile's own global slot (one barrier pair per
+// workgroup per block, no atomics).  A decision kernel later sums the tiles'
+// slots of a frame in a fixed order, so the energies, and with them every
+// early-exit decision, do not depend on the order in which workgroups run.
+// Halo waves hold no interior rows and never write: their LDS slots keep the
+// zeros written at the start of the tile.  A lane contributes half h of its
+// strip when those rows are interior rows of the tile and its columns are
+// interior columns.
 //
 // D <= 4 (transposed row sums): within each quad of lanes the D <= 4 partials
 // are exchanged so that lane q of the quad ends with the quad's sum of one
@@ -465,16 +480,29 @@ __device__ __forceinline__ float energy_block_sum(const float* block, int t, int
   return s;
 }
 
-// Add frames [t0, t0+nt) of the LDS block to energy (fp64, [T][D] from frame
-// t0).  All threads of the workgroup call it.
+// Per-tile energy slots in global memory: slot (t, u, tile) at
+// slots[(t * D + u) * stride + tile], fp32 (a tile's sum is formed in fp32 as
+// before; the sum over tiles is taken in fp64 by the decision kernel).
+// Store frames [t0, t0+nt) of the LDS block for `tile` (first frame t0; D
+// updates each).  All threads of the workgroup call it.
 template <int D, int BY>
-__device__ __forceinline__ void energy_flush(const float* block, int nt, double* energy_t0) {
+__device__ __forceinline__ void energy_store(const float* block, int nt, float* slots_t0,
+                                             int stride, int tile) {
   __syncthreads();
   for (int i = threadIdx.x; i < nt * D; i += blockDim.x) {
     const int t = i / D, u = i - t * D;
-    atomicAdd(energy_t0 + i, double(energy_block_sum<D, BY>(block, t, u)));
+    slots_t0[size_t(i) * stride + tile] = energy_block_sum<D, BY>(block, t, u);
   }
   __syncthreads();
+}
+
+// Sum over the tiles of one (frame, update) slot row in a fixed order: lane
+// i of the calling wave adds tiles i, i+64, ... in fp64, then a butterfly
+// (fixed lane pairing) gives every lane the total.  Wave-uniform result.
+__device__ __forceinline__ double slot_row_sum(const float* row, int ntiles) {
+  double s = 0.0;
+  for (int j = int(threadIdx.x & 63); j < ntiles; j += 64) s += double(row[j]);
+  return wave_sum(s);
 }
 
 // Early-exit iteration of one frame from its energies E_0..E_{D-1}

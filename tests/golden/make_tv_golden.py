@@ -143,6 +143,98 @@ def write_gap():
     print('gap cases:', len(meta))
 
 
+def legacy_A(f, Phi):
+    """Two-argument forward model the legacy solvers are called with."""
+    return np.sum(f * Phi, axis=2)
+
+
+def legacy_At(y, Phi):
+    return np.multiply(np.repeat(y[:, :, np.newaxis], Phi.shape[2], axis=2), Phi)
+
+
+def legacy_gap(y, Phi, Phi_sum, maxiter, step_size, weight, row, col):
+    """GAP_TV_rec (pnp_sci_algo.py:866-882) around scikit-image's TV, progress
+    printing left out: 30 TV iterations per call, y1 accumulated from zero."""
+    y1 = np.zeros((row, col))
+    f = legacy_At(y, Phi)
+    for _ in range(maxiter):
+        fb = legacy_A(f, Phi)
+        y1 = y1 + (y - fb)
+        f = f + np.multiply(step_size, legacy_At(np.divide(y1 - fb, Phi_sum), Phi))
+        f = denoise_tv_chambolle(f, weight, n_iter_max=30, multichannel=True)
+    return f
+
+
+def legacy_admm(y, Phi, Phi_sum, maxiter, step_size, weight, row, col, ColT, eta):
+    """ADMM_TV_rec (pnp_sci_algo.py:884-907) around scikit-image's TV: the TV
+    weight and eta decay by 0.999 and 0.998 after every iteration, and the
+    returned image is v (the last projection)."""
+    theta = legacy_At(y, Phi)
+    v = theta
+    b = np.zeros((row, col, ColT))
+    for _ in range(maxiter):
+        yb = legacy_A(theta + b, Phi)
+        v = (theta + b) + np.multiply(step_size, legacy_At(np.divide(y - yb, Phi_sum + eta), Phi))
+        theta = denoise_tv_chambolle(v - b, weight, n_iter_max=30, multichannel=True)
+        b = b - (v - theta)
+        weight = 0.999 * weight
+        eta = 0.998 * eta
+    return v
+
+
+def write_legacy():
+    """Legacy two-argument solvers with scikit-image's 30-iteration TV inside.
+
+    The TV weight is large enough that most frames run all 30 iterations and
+    some stop early on the energy test, so the fixtures record both branches;
+    ADMM runs long enough (25 iterations) for the decays to move the weight
+    by 2.5% and eta by 5%."""
+    rng = np.random.default_rng(2024)
+    arrays = {}
+    meta = []
+    k = 0
+    for (h, wd, t, iters, w, eta) in ((20, 24, 4, 6, 0.15, 0.01), (17, 31, 5, 25, 0.3, 0.05)):
+        for dt in (np.float32, np.float64):
+            truth = np.clip(np.stack([smooth_frame(rng, h, wd, 2.0) for _ in range(t)], -1), 0, 1)
+            truth += 0.05 * rng.standard_normal(truth.shape)
+            mask = (rng.random((h, wd, t)) < 0.5).astype(dt)
+            truth = truth.astype(dt)
+            y = np.sum(truth * mask, axis=2)
+            msum = np.sum(mask, axis=2)
+            msum[msum == 0] = 1
+            arrays['y_%d' % k] = y
+            arrays['mask_%d' % k] = mask
+            arrays['gap_%d' % k] = legacy_gap(y, mask, msum, iters, 1.0, w, h, wd)
+            arrays['admm_%d' % k] = legacy_admm(y, mask, msum, iters, 1.0, w, h, wd, t, eta)
+            meta.append((h, wd, t, iters, w, eta))
+            k += 1
+    arrays['meta'] = np.array(meta, dtype=np.float64)
+    np.savez_compressed(os.path.join(HERE, 'legacy_golden.npz'), **arrays)
+    print('legacy cases:', len(meta))
+
+
+def write_psnr():
+    """peak_signal_noise_ratio(ref, img, data_range=1.) as called at
+    pnp_sci_algo.py:702, on float32 and float64 pairs (scikit-image casts
+    both to float64 before the error)."""
+    from skimage.metrics import peak_signal_noise_ratio
+    rng = np.random.default_rng(11)
+    arrays = {}
+    vals = []
+    cases = [((16, 16), 0.1, np.float64), ((16, 16), 0.1, np.float32),
+             ((7, 9), 1e-3, np.float32), ((40, 33), 0.3, np.float64),
+             ((5, 6), 1e-6, np.float64), ((64, 64), 0.02, np.float32)]
+    for k, (shape, sigma, dt) in enumerate(cases):
+        ref = rng.random(shape).astype(dt)
+        img = (ref + sigma * rng.standard_normal(shape)).astype(dt)
+        arrays['ref_%d' % k] = ref
+        arrays['img_%d' % k] = img
+        vals.append(peak_signal_noise_ratio(ref, img, data_range=1.))
+    arrays['psnr'] = np.array(vals)
+    np.savez_compressed(os.path.join(HERE, 'psnr_golden.npz'), **arrays)
+    print('psnr cases:', len(vals))
+
+
 def write_ssim():
     """structural_similarity(X, Y, data_range=1.) as called at pnp_sci_algo.py:703."""
     from skimage.metrics import structural_similarity
@@ -163,6 +255,7 @@ def write_ssim():
 if __name__ == '__main__':
     if skimage.__version__ != '0.18.3':
         sys.exit('expected scikit-image 0.18.3, found %s' % skimage.__version__)
-    write_tv()
-    write_gap()
-    write_ssim()
+    which = sys.argv[1:] or ['tv', 'gap', 'ssim', 'legacy', 'psnr']
+    for name in which:
+        {'tv': write_tv, 'gap': write_gap, 'ssim': write_ssim, 'legacy': write_legacy,
+         'psnr': write_psnr}[name]()

@@ -251,3 +251,141 @@ def test_bayer_gap_equals_four_subgrid_solves():
     mse = np.mean((x - truth) ** 2)
     assert abs(psnr_all[-1] - 10 * np.log10(1 / mse)) < 1e-9
     assert len(psnr_all) == 4 and len(psnr_) == T and len(ssim_) == T
+
+
+# ---------------------------------------------------------------------------
+# Legacy solvers, PSNR and the coded-frame flips
+# ---------------------------------------------------------------------------
+
+def test_legacy_solvers_match_skimage_golden_bit_exact(golden_dir):
+    """GAP_TV_rec / ADMM_TV_rec (pnp_sci_algo.py:866-907) against fixtures
+    computed around scikit-image's own 30-iteration TV, with the 0.999/0.998
+    decays; float32 and float64."""
+    z = _golden(golden_dir, "legacy_golden.npz")
+    for k, (h, w, t, iters, wt, eta) in enumerate(z["meta"]):
+        h, w, t, iters = int(h), int(w), int(t), int(iters)
+        y, m = z["y_%d" % k], z["mask_%d" % k]
+        ps = O.phi_sum(m)
+        g = O.GAP_TV_rec(y, m, O.A_, O.At_, ps, iters, 1.0, float(wt), h, w, t, None)
+        a = O.ADMM_TV_rec(y, m, O.A_, O.At_, ps, iters, 1.0, float(wt), h, w, t, float(eta), None)
+        assert np.array_equal(g, z["gap_%d" % k]), ("gap", k)
+        assert np.array_equal(a, z["admm_%d" % k]), ("admm", k)
+
+
+def test_legacy_fixtures_exercise_exit_and_decay(golden_dir):
+    """The fixtures above only pin what they exercise: some 30-iteration TV
+    calls stop early on the energy test, and leaving out the decays (or
+    applying them before the first iteration) changes the ADMM result."""
+    z = _golden(golden_dir, "legacy_golden.npz")
+    k = 3  # 17x31x5, 25 iterations, float64
+    h, w, t, iters, wt, eta = z["meta"][k]
+    h, w, t, iters = int(h), int(w), int(t), int(iters)
+    y, m = z["y_%d" % k], z["mask_%d" % k]
+    ps = O.phi_sum(m)
+    want = z["admm_%d" % k]
+    # no decay: a plain ADMM loop with the same TV depth
+    nodecay = O.admm_denoise(y, ps, lambda v: O.A_(v, m), lambda v: O.At_(v, m),
+                             gamma=float(eta), iter_max=iters, tv_weight=float(wt),
+                             tv_iter_max=30, show_iqa=False)[0]
+    assert not np.allclose(nodecay, want, rtol=0, atol=1e-6)
+    # decays applied before the first use
+    early = O.ADMM_TV_rec(y, m, O.A_, O.At_, ps, iters, 1.0, 0.999 * float(wt), h, w, t,
+                          0.998 * float(eta), None)
+    assert not np.allclose(early, want, rtol=0, atol=1e-6)
+    # 30-iteration TV calls on these frames stop early on the energy test for
+    # some frames and run all 30 iterations for others
+    returned = []
+    for f in range(t):
+        st = {}
+        O.tv_chambolle_nd(want[..., f], float(wt), n_iter_max=30, _stats=st)
+        returned.append(st["returned"])
+    assert min(returned) < 29 <= max(returned), returned
+
+
+def test_legacy_gap_equals_generic_solver(golden_dir):
+    """GAP_TV_rec is gap_denoise (accelerated, lambda = step_size) with a
+    30-iteration TV and x0 = At(y): two code paths of the oracle meet the
+    same float64 fixtures.  (In float32 the legacy loop's y1 = zeros((row,
+    col)) is float64 and promotes the update, so the two differ there.)"""
+    z = _golden(golden_dir, "legacy_golden.npz")
+    for k, (h, w, t, iters, wt, eta) in enumerate(z["meta"]):
+        y, m = z["y_%d" % k], z["mask_%d" % k]
+        if y.dtype != np.float64:
+            continue
+        got = O.gap_denoise(y, O.phi_sum(m), lambda v: O.A_(v, m), lambda v: O.At_(v, m),
+                            iter_max=int(iters), tv_weight=float(wt), tv_iter_max=30,
+                            show_iqa=False)[0]
+        assert np.array_equal(got, z["gap_%d" % k]), k
+
+
+def test_psnr_matches_skimage_golden(golden_dir):
+    """compare_psnr against scikit-image's peak_signal_noise_ratio values, and
+    psnr (utils.py:28-36, 20*log10(1/sqrt(mse)) in the input dtype) against
+    the same values: equal in float64, within float32 rounding otherwise."""
+    z = _golden(golden_dir, "psnr_golden.npz")
+    for k, want in enumerate(z["psnr"]):
+        ref, img = z["ref_%d" % k], z["img_%d" % k]
+        assert abs(O.compare_psnr(ref, img, data_range=1.) - want) < 1e-12, k
+        tol = 1e-10 if ref.dtype == np.float64 else 1e-4
+        assert abs(O.psnr(ref, img) - want) < tol, (k, O.psnr(ref, img), want)
+
+
+def test_psnr_closed_forms():
+    # mse = 0.01 exactly -> 20 dB; mse = 1e-4 -> 40 dB; identical -> 100 (utils)
+    # and inf (scikit-image); data_range scales as 20*log10(range).
+    ref = np.zeros((4, 5))
+    img = np.full((4, 5), 0.1)
+    assert O.psnr(ref, img) == pytest.approx(20.0, abs=1e-12)
+    assert O.psnr(ref, img / 10) == pytest.approx(40.0, abs=1e-12)
+    assert O.psnr(ref, ref) == 100
+    assert O.compare_psnr(ref, ref) == np.inf
+    assert O.compare_psnr(ref, img, data_range=255.) == pytest.approx(
+        20.0 + 20 * np.log10(255.), abs=1e-9)
+    # the mean runs over every element, not one frame
+    img3 = np.zeros((4, 5, 2))
+    img3[..., 1] = 0.1
+    assert O.psnr(np.zeros((4, 5, 2)), img3) == pytest.approx(10 * np.log10(200.), abs=1e-12)
+
+
+@pytest.mark.parametrize("direction", ["updown", "downup"])
+def test_cacti_flips_odd_even_blocks(direction):
+    """admmdenoise_cacti (pnp_sci_algo.py:504-510,521-526): under 'updown' the
+    odd coded frames, under 'downup' the even ones, solve with the warm start
+    reversed along frames and return the result reversed (with its per-frame
+    PSNR/SSIM lists); the mask and the truth are not reversed."""
+    rng = np.random.default_rng(8)
+    H, W, T, F = 10, 12, 4, 3
+    mask = (rng.random((H, W, T)) < 0.5).astype(np.float64)
+    orig = rng.random((H, W, T * F)) * 255
+    meas = np.stack([O.A_(orig[..., f * T:(f + 1) * T], mask) for f in range(F)], -1)
+    v0 = rng.random((H, W, T * F))
+    A = lambda v: O.A_(v, mask)  # noqa: E731
+    At = lambda v: O.At_(v, mask)  # noqa: E731
+    args = dict(denoiser="tv", iter_max=3, tv_weight=0.1, tv_iter_max=3, show_iqa=False)
+    x_, _, p_, s_, pall = O.admmdenoise_cacti(meas, mask, A, At, projmeth="admm", v0=v0,
+                                              orig=orig, nframe=F, MAXB=255.,
+                                              maskdirection=direction, **args)
+    ps = O.phi_sum(mask)
+    for kf in range(F):
+        flip = (kf % 2 == 1) if direction == "updown" else (kf % 2 == 0)
+        v0k = v0[..., kf * T:(kf + 1) * T]
+        xk, pk, sk, ak = O.admm_denoise(meas[..., kf] / 255., ps, A, At,
+                                        x0=v0k[..., ::-1] if flip else v0k,
+                                        X_orig=orig[..., kf * T:(kf + 1) * T] / 255., **args)
+        if flip:
+            xk, pk, sk, ak = xk[..., ::-1], pk[::-1], sk[::-1], ak[::-1]
+        assert np.array_equal(x_[..., kf * T:(kf + 1) * T], xk.astype(np.float32)), kf
+        assert p_[kf * T:(kf + 1) * T] == list(pk) and s_[kf * T:(kf + 1) * T] == list(sk)
+        assert list(pall[kf]) == list(ak)
+        # a flipped block differs from the unflipped solve (the flip is not a no-op)
+        if flip:
+            plain = O.admm_denoise(meas[..., kf] / 255., ps, A, At, x0=v0k, **args)[0]
+            assert not np.allclose(plain, xk[..., ::-1])
+    # iframe shifts the parity: block 0 of iframe=1 is an odd coded frame
+    x1 = O.admmdenoise_cacti(meas, mask, A, At, projmeth="admm", v0=v0[..., T:2 * T],
+                             nframe=1, iframe=1, MAXB=255., maskdirection=direction, **args)[0]
+    k1 = O.admm_denoise(meas[..., 1] / 255., ps, A, At,
+                        x0=(v0[..., T:2 * T][..., ::-1] if direction == "updown"
+                            else v0[..., T:2 * T]), **args)[0]
+    want = k1[..., ::-1] if direction == "updown" else k1
+    assert np.array_equal(x1, want.astype(np.float32))
