@@ -187,7 +187,9 @@ size_t scigap_solve_workspace_bytes(const scigap_params* p);
  *                    k (device array of `iters` doubles), the quantity behind
  *                    psnr_all (pnp_sci_algo.py:681-682)
  *   stats    optional device array of `iters` ints: per iteration, the
- *            number of frames whose TV stopped early on the energy test.
+ *            number of frames whose TV stopped early on the energy test
+ *            (ADMM: 0 in the last iteration, whose TV result never reaches
+ *            the returned x and is not computed).
  * x0 may not overlap x_out.  The call is asynchronous on `stream`. */
 int scigap_solve(const scigap_params* p,
                  const float* y, const float* phi, const float* phi_sum,

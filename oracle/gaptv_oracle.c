@@ -131,19 +131,22 @@ void oracle_tv(const double* img, double* out, int H, int W, int T, double weigh
  * method: 0 accelerated GAP, 1 plain GAP, 2 ADMM.  x0 may be NULL (x0 = At(y)).
  * x (T,H,W) receives the result: GAP returns x after the last TV, ADMM the
  * last projection x.  psnr_sq (optional, `iters` doubles) receives
- * sum((truth - x_k)^2) per iteration when truth is given.
+ * sum((truth - x_k)^2) per iteration when truth is given; exits (optional,
+ * `iters` ints) the number of frames whose TV stopped on the energy test
+ * before tv_iter_max - 1 updates in each iteration.
  */
 void oracle_gap_tv(const double* y, const double* phi, const double* phi_sum, const double* x0,
                    double* x, int H, int W, int T, int method, int clip, int iters, double lam,
                    double gamma, double tv_weight, double eps, int tv_iter_max,
                    double weight_decay, double gamma_decay, const double* truth,
-                   double* sq_err) {
+                   double* sq_err, int* exits) {
   const size_t N = (size_t)H * W, V = N * (size_t)T;
   double* yb = (double*)malloc(N * sizeof(double));
   double* y1 = (double*)calloc(N, sizeof(double));
   double* v = (double*)malloc(V * sizeof(double));
   double* theta = (double*)malloc(V * sizeof(double));
   double* b = (double*)calloc(V, sizeof(double));
+  int* ret = (int*)malloc((size_t)T * sizeof(int));
   /* x = theta = x0 (or At(y)) */
 #pragma omp parallel for
   for (long long i = 0; i < (long long)N; ++i)
@@ -170,7 +173,7 @@ void oracle_gap_tv(const double* y, const double* phi, const double* phi_sum, co
           v[o] = x[o] - b[o];
         }
       }
-      oracle_tv(v, theta, H, W, T, weight, eps, tv_iter_max, NULL);
+      oracle_tv(v, theta, H, W, T, weight, eps, tv_iter_max, ret);
 #pragma omp parallel for
       for (long long o = 0; o < (long long)V; ++o) {
         if (clip) theta[o] = theta[o] < 0.0 ? 0.0 : (theta[o] > 1.0 ? 1.0 : theta[o]);
@@ -193,7 +196,12 @@ void oracle_gap_tv(const double* y, const double* phi, const double* phi_sum, co
           v[o] = x[o] + lam * (r * phi[o]);
         }
       }
-      oracle_tv(v, x, H, W, T, weight, eps, tv_iter_max, NULL);
+      oracle_tv(v, x, H, W, T, weight, eps, tv_iter_max, ret);
+    }
+    if (exits) {
+      int n = 0;
+      for (int t = 0; t < T; ++t) n += ret[t] < tv_iter_max - 1;
+      exits[k] = n;
     }
     if (truth && sq_err) {
       double s = 0.0;
@@ -212,4 +220,5 @@ void oracle_gap_tv(const double* y, const double* phi, const double* phi_sum, co
   free(v);
   free(theta);
   free(b);
+  free(ret);
 }

@@ -21,7 +21,7 @@ _dll.oracle_tv.argtypes = [_D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int, cty
 _dll.oracle_gap_tv.argtypes = [_D, _D, _D, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
-                               ctypes.c_double, ctypes.c_double, _D, _D]
+                               ctypes.c_double, ctypes.c_double, _D, _D, _I]
 
 METHODS = {"gap": 0, "gap_plain": 1, "admm": 2}
 
@@ -47,9 +47,10 @@ def tv(image, weight=0.1, eps=2e-4, n_iter_max=200):
 
 def gap_tv(y, mask, phi_sum, iters=50, method="gap", _lambda=1.0, gamma=0.01, clip=False,
            tv_weight=0.1, eps=2e-4, tv_iter_max=5, x0=None, truth=None, weight_decay=1.0,
-           gamma_decay=1.0):
+           gamma_decay=1.0, exits=False):
     """FP64 GAP/ADMM-TV; returns x (H,W,T), plus sum of squared errors per
-    iteration when `truth` is given."""
+    iteration when `truth` is given, plus (last) the number of frames whose TV
+    stopped early in each iteration when `exits`."""
     phi = _thw(mask)
     T, H, W = phi.shape
     yy = np.ascontiguousarray(y, np.float64)
@@ -58,9 +59,15 @@ def gap_tv(y, mask, phi_sum, iters=50, method="gap", _lambda=1.0, gamma=0.01, cl
     tr = _thw(truth) if truth is not None else None
     x = np.empty((T, H, W), dtype=np.float64)
     sq = np.zeros(iters, dtype=np.float64) if truth is not None else None
+    ex = np.zeros(max(iters, 1), dtype=np.int32)
     _dll.oracle_gap_tv(_p(yy), _p(phi), _p(ps), _p(xx0), _p(x), H, W, T, METHODS[method],
                        int(clip), int(iters), float(_lambda), float(gamma), float(tv_weight),
                        float(eps), int(tv_iter_max), float(weight_decay), float(gamma_decay),
-                       _p(tr), _p(sq))
+                       _p(tr), _p(sq), ex.ctypes.data_as(_I))
     xo = np.moveaxis(x, 0, -1)
-    return (xo, sq) if truth is not None else xo
+    res = (xo,)
+    if truth is not None:
+        res += (sq,)
+    if exits:
+        res += (ex[:iters],)
+    return res if len(res) > 1 else xo

@@ -452,7 +452,9 @@ def admmdenoise_cacti(meas, mask, A, At, projmeth='admm', v0=None, orig=None,
         elif projmeth.lower() == 'gap':
             res = gap_denoise(meas_k, msum, A, At, x0=v0_k, X_orig=orig_k, **args)
         else:
-            raise ValueError('Unsupported projection method %s' % projmeth.upper())
+            # pnp_sci_algo.py:518-529: printed, then the unbound x_k is read
+            print('Unsupported projection method %s' % projmeth.upper())
+            raise UnboundLocalError("local variable 'x_k' referenced before assignment")
         x_k, psnr_k, ssim_k, psnrall_k = res
         if flip:
             x_k = x_k[..., ::-1]

@@ -179,7 +179,7 @@ struct BatchLayout {
   int2* items = nullptr;        // all items (repair pass)
   int* counters = nullptr;      // [fused_iters][8]
   int* any = nullptr;           // [fused_iters]
-  int* zeroed = nullptr;        // zeroed before every run: per solve nexit [K], then alt[0] [T]
+  int* zeroed = nullptr;        // zeroed before every run: nexit [K] of every solve
   int zeroed_count = 0;
   int nitems = 0;
   int maxT = 0;
@@ -206,7 +206,7 @@ BatchLayout layout(const scigap_params* p, const scigap_batch_item* items, int n
   };
   int total_tiles = 0;
   size_t zc = 0;
-  for (int i = 0; i < n; ++i) zc += size_t(K) + (D >= 2 ? size_t(items[i].T) : 0);
+  for (int i = 0; i < n; ++i) zc += size_t(K);
   L.zeroed = reinterpret_cast<int*>(take(zc * 4));
   L.zeroed_count = int(zc);
   size_t zoff = 0;
@@ -226,13 +226,6 @@ BatchLayout layout(const scigap_params* p, const scigap_batch_item* items, int n
     const int tiles = batch_tiles(items[i].H, items[i].W, g, nullptr);
     s.eslot_stride = tiles;
     s.eslot = reinterpret_cast<float*>(take(T * D * size_t(tiles) * 4));
-    if (D >= 2) {
-      // frames kept after D-1 updates (flags of iteration 0 are zeroed)
-      s.xalt = reinterpret_cast<float*>(take(T * N * 4));
-      s.alt[0] = L.zeroed ? L.zeroed + zoff : nullptr;
-      zoff += T;
-      s.alt[1] = reinterpret_cast<int*>(take(T * 4));
-    }
     total_tiles += tiles;
     L.maxT = std::max(L.maxT, int(T));
   }

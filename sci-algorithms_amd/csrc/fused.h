@@ -42,10 +42,6 @@ struct FusedArgs {
   int* nexit;           // frames of this iteration that stopped early (k_decide adds;
                         // zero before the iteration)
   int* stats;           // optional copy of *nexit for the caller (k_decide adds)
-  float* x_alt;         // optional (T,H,W): the image after D-1 updates, written by the
-                        // main pass for the frames flagged in alt_in
-  const int* alt_in;    // [T] flags of this iteration (x_alt holds the frame), or null
-  int* alt_out;         // [T] flags for the next iteration (k_decide), or null
   int H, W, T;
   float lam, gam_next, weight, tw;
   double eps;
@@ -456,25 +452,11 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
 
     float out[RY][2];
     float acc[NH * 8];
-    // Frames flagged for this iteration (their last exit test was close; see
-    // k_decide) also keep the image after D-1 updates, so that a stop there
-    // costs the repair a load instead of a recomputation.
-    float* xa = nullptr;
-    if constexpr (D >= 2 && MODE != TV_ONLY)
-      if (a.alt_in && a.alt_in[t_lo + t]) xa = a.x_alt + size_t(t_lo + t) * N;  // uniform
-    auto save = [&](const float (&o)[RY][2]) {
-      if (xa && owns_any && tp.own_c[0]) {
-        unsigned base = unsigned(off0) * 4u;
-        compute_here(base);
-        store_owned<D, BY, MR, INSIDE>(xa, N, W, tp, base, o);
-      }
-    };
     if constexpr (X1)
-      tv_run_x1<D, RY, BY, NH, !INSIDE>(img, out, D, lds, lp, em, a.weight, a.tw, acc, true,
-                                        save);
+      tv_run_x1<D, RY, BY, NH, !INSIDE>(img, out, D, lds, lp, em, a.weight, a.tw, acc, true);
     else
       tv_run_from_zero<D, RY, BY, NH, !INSIDE>(img, out, D, lds, lp, em, a.weight, a.tw, acc,
-                                               true, save);
+                                               true);
 
     if (owns_any) {
       float bv[RY][2];
@@ -621,35 +603,50 @@ __global__ void __launch_bounds__(64 * BY) k_fused(FusedArgs a) {
   }
 }
 
-// Flag for the next iteration: keep the image after D-1 updates of a frame
-// whose exit test at D-1 passed or came within kAltMargin times its threshold.
-constexpr double kAltMargin = 4.0;
-
 // Early-exit decision of one frame (workgroup blockIdx.x; 256 threads): the
 // energies E_0..E_{D-1} are the sums of the frame's per-tile slots, taken in
-// a fixed order (slot_row_sum), then the exit test of _denoise.py:384-391.
-// Writes stop[t], adds early stops to *nexit and *stats, and the x_alt flag
-// of the frame for the next iteration.
+// a fixed order, then the exit test of _denoise.py:384-391.  Thread i adds
+// tiles i, i+256, ... of every update row with all of its loads issued
+// together (one memory latency per frame, not one per tile), in fp64; the
+// waves then reduce by a fixed butterfly and wave 0 adds the four wave sums
+// in order.  Writes stop[t] and adds early stops to *nexit and *stats.
 template <int D>
 __device__ __forceinline__ void decide_frame(const FusedArgs& a, int t) {
-  __shared__ double e[D];
+  __shared__ double part[4][D];
+  const float* rows = a.eslot + size_t(t) * D * a.eslot_stride;
+  double acc[D];
+#pragma unroll
+  for (int u = 0; u < D; ++u) acc[u] = 0.0;
+  constexpr int kUnroll = 4;
+  for (int j0 = int(threadIdx.x); j0 < a.eslot_tiles; j0 += 256 * kUnroll) {
+    float v[kUnroll][D];
+#pragma unroll
+    for (int q = 0; q < kUnroll; ++q) {
+      const int j = j0 + q * 256;
+#pragma unroll
+      for (int u = 0; u < D; ++u) v[q][u] = j < a.eslot_tiles ? rows[size_t(u) * a.eslot_stride + j] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < kUnroll; ++q)
+#pragma unroll
+      for (int u = 0; u < D; ++u) acc[u] += double(v[q][u]);
+  }
   const int w = threadIdx.x >> 6;
-  for (int u = w; u < D; u += 4) {  // wave-uniform
-    const double v = slot_row_sum(a.eslot + (size_t(t) * D + u) * a.eslot_stride, a.eslot_tiles);
-    if ((threadIdx.x & 63) == 0) e[u] = v;
+#pragma unroll
+  for (int u = 0; u < D; ++u) {
+    const double sum = wave_sum(acc[u]);
+    if ((threadIdx.x & 63) == 0) part[w][u] = sum;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    double e[D];
+#pragma unroll
+    for (int u = 0; u < D; ++u) e[u] = ((part[0][u] + part[1][u]) + part[2][u]) + part[3][u];
     const int st = exit_iteration<D>(e, a.eps);
     a.stop[t] = st;
     if (st < D) {
       atomicAdd(a.nexit, 1);
       if (a.stats) atomicAdd(a.stats, 1);
-    }
-    if (a.alt_out) {
-      int flag = 0;
-      if constexpr (D >= 2) flag = fabs(e[D - 2] - e[D - 1]) < kAltMargin * a.eps * e[0];
-      a.alt_out[t] = flag;
     }
   }
 }
@@ -661,12 +658,10 @@ __global__ void __launch_bounds__(256) k_decide(FusedArgs a) {
 
 // Early-exit repair (does work only when some frame stopped early; `stop`
 // holds each frame's exit iteration, D for none).
-// For every frame that stopped at s < D, the frame's image after s updates
-// replaces the main pass's: loaded from x_alt when s = D-1 and the main pass
-// kept it, else recomputed with s updates from the unchanged inputs (x_in
-// and r_in are double-buffered).  Its contribution to yb is corrected by
-// Phi*(x_new - x_old) (ADMM: 2*Phi*...) and the residual epilogue is
-// recomputed from the corrected yb.
+// For every frame that stopped at s < D, the frame is recomputed with s
+// updates from the unchanged inputs (x_in and r_in are double-buffered), its
+// contribution to yb is corrected by Phi*(x_new - x_old) (ADMM: 2*Phi*...),
+// and the residual epilogue is recomputed from the corrected yb.
 template <int D, int BY, int MR>
 struct FixLds {
   TileLds<BY> eng;
@@ -713,21 +708,10 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
   float dyb[RY][2] = {};
   float err = 0.f;
 
-  // A stopped frame's image comes from x_alt when it stopped after D-1
-  // updates and the main pass kept that image (wave-uniform per frame).
-  auto from_alt = [&](int t) {
-    if constexpr (kTvOnly || D < 2) return false;
-    return stop[t] == D - 1 && a.alt_in && a.alt_in[t] != 0;
-  };
-  // The TV input img = x + q*Phi is needed for a recomputation and for
-  // ADMM's b = x_new - v.
-  auto needs_x = [&](int t) {
-    return !from_y && (!from_alt(t) || (MODE == ADMM && a.write_b));
-  };
   // The stopped frames are visited in order.  A frame's x and Phi are
   // requested one stopped frame ahead and its old output before its TV, so
   // that their latency hides behind the stencil arithmetic (as in the main
-  // pass); nx stays 0 when X_0 = At(y) or when x is not needed.
+  // pass); nx stays 0 when X_0 = At(y).
   auto next_stopped = [&](int t) {
     while (t < T && stop[t] >= D) ++t;
     return t;
@@ -736,13 +720,12 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
   float nph[RY][2] = {}, nx[RY][2] = {};
   if (t < T) {
     if constexpr (!kTvOnly) load_plane<RY, INSIDE>(a.phi + size_t(t) * N, nph, lp, off0, H, W);
-    if (needs_x(t)) load_plane<RY, INSIDE>(a.x_in + size_t(t) * N, nx, lp, off0, H, W);
+    if (!from_y) load_plane<RY, INSIDE>(a.x_in + size_t(t) * N, nx, lp, off0, H, W);
   }
   // Every lane reads back only the LDS slots it wrote itself.
   float2* const simg = sh.img + lp.w * RY * 64 + lp.lane;
   while (t < T) {
     const int nup = stop[t];
-    const bool alt = from_alt(t);
     float ph[RY][2];
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
@@ -755,20 +738,15 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
     float* xo = a.x_out + size_t(t) * N;
     float xold[RY][2] = {};
     if constexpr (!kTvOnly) load_plane<RY, INSIDE>(xo, xold, lp, off0, H, W);
-    float out[RY][2];
-    if (alt) load_plane<RY, INSIDE>(a.x_alt + size_t(t) * N, out, lp, off0, H, W);
     const int tn = next_stopped(t + 1);
     if (tn < T) {
-#pragma unroll
-      for (int j = 0; j < RY; ++j) nx[j][0] = nx[j][1] = 0.f;
       if constexpr (!kTvOnly) load_plane<RY, INSIDE>(a.phi + size_t(tn) * N, nph, lp, off0, H, W);
-      if (needs_x(tn)) load_plane<RY, INSIDE>(a.x_in + size_t(tn) * N, nx, lp, off0, H, W);
+      if (!from_y) load_plane<RY, INSIDE>(a.x_in + size_t(tn) * N, nx, lp, off0, H, W);
     }
-    if (!alt) {
-      float acc[NH * 8];
-      tv_run_from_zero<D, RY, BY, NH, !INSIDE>(simg, out, nup, lds, lp, em, a.weight, a.tw, acc,
-                                               false);
-    }
+    float out[RY][2];
+    float acc[NH * 8];
+    tv_run_from_zero<D, RY, BY, NH, !INSIDE>(simg, out, nup, lds, lp, em, a.weight, a.tw, acc,
+                                             false);
     const float* trt = a.truth ? a.truth + size_t(t) * N : nullptr;
     float* bt = a.b ? a.b + size_t(t) * N : nullptr;
 #pragma unroll
@@ -814,6 +792,10 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
     if (a.truth && a.err) commit_err<BY>(err, a.err, red64, lp.w, lp.lane);
 }
 
+// Frames whose exit iterations the repair kernel copies into LDS (the frame
+// scans then cost no memory latency); longer solves read them from memory.
+constexpr int kStopLds = 1024;
+
 // For D <= 4 at least 4 waves per SIMD: two 8-wave workgroups per CU
 // (128 VGPRs); deeper tiles need more registers than that.  Runs after
 // k_decide; returns at once when no frame stopped early (the common case).
@@ -821,14 +803,21 @@ template <int D, int BY, int MR, int MODE>
 __global__ void __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(D <= 4 ? 4 : 1)))
 k_fused_fix(FusedArgs a) {
   __shared__ FixLds<D, BY, MR> sh;
+  __shared__ int s_stop[kStopLds];
   if (*a.nexit == 0) return;  // one scalar load per workgroup
   const int tile = tile_of_block(blockIdx.x, a.per_xcd);
   if (tile >= a.ntiles) return;
+  const int* stop = a.stop;
+  if (a.T <= kStopLds) {
+    for (int t = threadIdx.x; t < a.T; t += blockDim.x) s_stop[t] = a.stop[t];
+    __syncthreads();
+    stop = s_stop;
+  }
   // Interior tiles run without masks or bounds checks (as in the main pass).
   if (tile_inside<D, BY, MR>(tile, a.tiles_x, a.H, a.W))
-    fused_fix_tile<D, BY, MR, MODE, true>(a, tile, sh, a.stop);
+    fused_fix_tile<D, BY, MR, MODE, true>(a, tile, sh, stop);
   else
-    fused_fix_tile<D, BY, MR, MODE, false>(a, tile, sh, a.stop);
+    fused_fix_tile<D, BY, MR, MODE, false>(a, tile, sh, stop);
 }
 
 // Initial residual: X_0 = x0 (or At(y)), yb = A(X_0), then
@@ -1025,8 +1014,6 @@ struct SolveState {
   int* nexit = nullptr;      // [iters]
   float* eslot = nullptr;    // [T][D][eslot_stride] per-tile energies (reused every iteration)
   int eslot_stride = 0;
-  float* xalt = nullptr;     // (T,H,W) image after D-1 updates of flagged frames, or null
-  int* alt[2] = {nullptr, nullptr};  // [T] flags, ping-pong between iterations
 };
 
 inline int solver_mode(const scigap_params& p) {
@@ -1072,9 +1059,6 @@ inline FusedArgs iteration_args(const scigap_params& p, int k, int64_t H, int64_
   a.b = st.bbuf;
   a.eslot = st.eslot;
   a.eslot_stride = st.eslot_stride;
-  a.x_alt = st.xalt;
-  a.alt_in = st.xalt ? st.alt[k & 1] : nullptr;
-  a.alt_out = st.xalt ? st.alt[(k + 1) & 1] : nullptr;
   a.yb = st.yb;
   a.yb_part = st.yb_part;
   a.stop = st.stop;

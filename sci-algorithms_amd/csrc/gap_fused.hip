@@ -77,9 +77,9 @@ unsigned set_tiles(FusedArgs& a) {
   return unsigned(a.per_xcd * 8);
 }
 
-template <int D, int BY, int MODE, bool IQA, bool X1, int PART, bool DMA = false>
+template <int D, int BY, int MODE, bool IQA, bool X1, int PART, bool DMA = false, int MR = 1>
 void launch_kernel(const FusedArgs& a, unsigned grid, hipStream_t s) {
-  k_fused<D, BY, 1, MODE, IQA, X1, PART, DMA><<<grid, 64 * BY, 0, s>>>(a);
+  k_fused<D, BY, MR, MODE, IQA, X1, PART, DMA><<<grid, 64 * BY, 0, s>>>(a);
 }
 
 // Engines of the fused kernel (scigap_params.engine):
@@ -101,7 +101,7 @@ constexpr int kMaxParts = 8;  // frame parts (yb_part planes in the workspace)
 // Workgroups of the fused kernel <D, BY, MODE, IQA> resident on the whole
 // device (queried once per instantiation; every device of a process is
 // assumed to be the same model).
-template <int D, int BY, int MODE, bool IQA>
+template <int D, int BY, int MODE, bool IQA, int MR = 1>
 int resident_workgroups() {
   static std::atomic<int> cached{0};  // host threads may race to fill it (same value)
   int v = cached.load(std::memory_order_relaxed);
@@ -110,7 +110,7 @@ int resident_workgroups() {
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, k_fused<D, BY, 1, MODE, IQA, false, 0>, 64 * BY, 0) != hipSuccess ||
+            &per_cu, k_fused<D, BY, MR, MODE, IQA, false, 0>, 64 * BY, 0) != hipSuccess ||
         cus <= 0 || per_cu <= 0)
       return 256;
     v = cus * per_cu;
@@ -135,14 +135,14 @@ int pick_parts(int tiles, int T, int slots) {
   return best;
 }
 
-template <int D, int BY, int MODE, bool IQA>
+template <int D, int BY, int MODE, bool IQA, int MR = 1>
 int launch_main(const FusedArgs& a0, hipStream_t s, int engine) {
   FusedArgs a = a0;
-  const unsigned grid = set_tiles<D, BY, 1>(a);
+  const unsigned grid = set_tiles<D, BY, MR>(a);
   if (engine == 0) engine = kDefaultEngine;
   int parts = 1;
   if (engine == 6 && (MODE == TV_ONLY || a.yb_part)) {
-    const int slots = resident_workgroups<D, BY, MODE, IQA>();
+    const int slots = resident_workgroups<D, BY, MODE, IQA, MR>();
     parts = pick_parts(a.ntiles, a.T, slots);
   }
   if (parts > 1) {
@@ -150,7 +150,7 @@ int launch_main(const FusedArgs& a0, hipStream_t s, int engine) {
     a.ntiles *= parts;
     a.per_xcd = (a.ntiles + 7) / 8;
     ProfScope ps(MODE == TV_ONLY ? "tv_fused" : "fused_iteration", s);
-    launch_kernel<D, BY, MODE, IQA, false, 0>(a, unsigned(a.per_xcd * 8), s);
+    launch_kernel<D, BY, MODE, IQA, false, 0, false, MR>(a, unsigned(a.per_xcd * 8), s);
     if constexpr (MODE != TV_ONLY) {
       const int64_t N = int64_t(a.H) * a.W;
       const unsigned g = unsigned(std::min<int64_t>((N + 255) / 256, 4096));
@@ -187,35 +187,36 @@ int launch_main(const FusedArgs& a0, hipStream_t s, int engine) {
   return SCIGAP_OK;
 }
 
-template <int D, int BY>
+template <int D, int BY, int MR = 1>
 int launch_mode(const FusedArgs& a, hipStream_t s, int engine) {
   const bool iqa = a.truth != nullptr;
   if (a.mode == GAP_ACC)
-    return iqa ? launch_main<D, BY, GAP_ACC, true>(a, s, engine)
-               : launch_main<D, BY, GAP_ACC, false>(a, s, engine);
+    return iqa ? launch_main<D, BY, GAP_ACC, true, MR>(a, s, engine)
+               : launch_main<D, BY, GAP_ACC, false, MR>(a, s, engine);
   if (a.mode == GAP_PLAIN)
-    return iqa ? launch_main<D, BY, GAP_PLAIN, true>(a, s, engine)
-               : launch_main<D, BY, GAP_PLAIN, false>(a, s, engine);
+    return iqa ? launch_main<D, BY, GAP_PLAIN, true, MR>(a, s, engine)
+               : launch_main<D, BY, GAP_PLAIN, false, MR>(a, s, engine);
   if (a.mode == ADMM)
-    return iqa ? launch_main<D, BY, ADMM, true>(a, s, engine)
-               : launch_main<D, BY, ADMM, false>(a, s, engine);
-  return launch_main<D, BY, TV_ONLY, false>(a, s, engine);
+    return iqa ? launch_main<D, BY, ADMM, true, MR>(a, s, engine)
+               : launch_main<D, BY, ADMM, false, MR>(a, s, engine);
+  return launch_main<D, BY, TV_ONLY, false, MR>(a, s, engine);
 }
 
-// Tiles of the main launch for (D, BY): the energy slots' row length.
-template <int D, int BY>
+// Tiles of the main launch for (D, BY, MR): the energy slots' row length.
+template <int D, int BY, int MR = 1>
 int main_tiles(int H, int W) {
-  using G = TileGeom<D, BY, 1>;
-  return tiles_x_of<D, BY, 1>(W) * ((H + G::OY - 1) / G::OY);
+  using G = TileGeom<D, BY, MR>;
+  return tiles_x_of<D, BY, MR>(W) * ((H + G::OY - 1) / G::OY);
 }
 
 template <int D>
-int main_tiles_for(int H, int W, int tile_rows) {
-  if constexpr (D == 4)
+int main_tiles_for(int H, int W, int tile_rows, int engine) {
+  if constexpr (D == 4) {
     return tile_rows == 12 ? main_tiles<4, 12>(H, W)
            : tile_rows == 16 ? main_tiles<4, 16>(H, W) : main_tiles<4, 8>(H, W);
-  else
+  } else {
     return main_tiles<D, FusedShape<D>::BY>(H, W);
+  }
 }
 
 template <int D>
@@ -223,7 +224,7 @@ int launch_fused(const FusedArgs& a00, hipStream_t s, int tile_rows, int engine)
   constexpr int BY = FusedShape<D>::BY, MR = FusedShape<D>::MR;
   FusedArgs a0 = a00;
   if (a0.eslot) {
-    a0.eslot_tiles = main_tiles_for<D>(a0.H, a0.W, tile_rows);
+    a0.eslot_tiles = main_tiles_for<D>(a0.H, a0.W, tile_rows, engine);
     if (a0.eslot_tiles > a0.eslot_stride)
       return fail(SCIGAP_EWORKSPACE, "energy slots hold %d tiles, the launch has %d",
                   a0.eslot_stride, a0.eslot_tiles);
@@ -326,10 +327,6 @@ Workspace carve(const scigap_params* p, char* base) {
   if (p->tv_iter_max - 1 >= 1 && p->tv_iter_max - 1 <= kMaxFused) {
     w.st.eslot_stride = max_slot_tiles(D, int(p->H), int(p->W));
     w.st.eslot = reinterpret_cast<float*>(take(T * D * size_t(w.st.eslot_stride) * 4));
-    if (D >= 2) {
-      w.st.xalt = reinterpret_cast<float*>(take(T * N * 4));
-      for (int i = 0; i < 2; ++i) w.st.alt[i] = reinterpret_cast<int*>(take(T * 4));
-    }
   }
   w.st.yb = reinterpret_cast<float*>(take(N * 4));
   w.st.yb_part = reinterpret_cast<float*>(take(std::min<size_t>(T, kMaxParts) * N * 4));
@@ -350,7 +347,8 @@ int check_params(const scigap_params* p) {
   if (int rc = check_shape(p->H, p->W, p->T)) return rc;
   if (p->method != SCIGAP_METHOD_GAP && p->method != SCIGAP_METHOD_ADMM)
     return fail(SCIGAP_EINVAL, "unknown method %d", p->method);
-  if (p->iters < 0) return fail(SCIGAP_EINVAL, "iters must be >= 0");
+  if (p->iters < 0 || p->iters > (1 << 24))
+    return fail(SCIGAP_EINVAL, "iters must be in [0, 2^24]");
   if (p->tv_iter_max < 1) return fail(SCIGAP_EINVAL, "tv_iter_max must be >= 1");
   if (!(p->tv_weight > 0.f) || !std::isfinite(p->tv_weight))
     return fail(SCIGAP_EINVAL, "tv_weight must be finite and > 0");
@@ -484,13 +482,9 @@ int scigap_solve(const scigap_params* p, const float* y, const float* phi, const
     return scigap_adjoint(y, phi, x_out, p->H, p->W, T, stream);
   }
   if (!deep && D > 0) {
-    // early-stop counters of every iteration; no frame kept for iteration 0
+    // early-stop counters of every iteration
     k_fill_i32<<<unsigned((K + 255) / 256), 256, 0, s>>>(ws.st.nexit, K, 0);
     SCIGAP_LAUNCH_CHECK("counter fill");
-    if (ws.st.alt[0]) {
-      k_fill_i32<<<unsigned((T + 255) / 256), 256, 0, s>>>(ws.st.alt[0], int(T), 0);
-      SCIGAP_LAUNCH_CHECK("flag fill");
-    }
   }
   if (truth)
     if (int rc = fill_f64(sq_err, K, 0.0, s)) return rc;

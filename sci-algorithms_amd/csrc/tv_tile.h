@@ -186,26 +186,17 @@ __device__ __forceinline__ void tv_grad(const float (&out)[RY][2], float (&p0)[R
     }
 }
 
-// Callback of the engines, handed the image after D-1 updates (before the
-// last one); the default does nothing.
-struct NoSave {
-  template <class V>
-  __device__ void operator()(const V&) const {}
-};
-
 // Run `nup` (<= D) updates starting from p = 0 (out = img).  `img` points at
 // this lane's float2 of register row 0 in an LDS plane laid out [row][64 lanes]
 // (row j at img[j*64]).  On return `out` holds the image after nup updates and
 // acc.e[h][u] this lane's partial of E_u over half h of its strip.  When
 // `accumulate` is false the energies are not formed (recomputation passes).
-// Energy slots are addressed as e[h][u] = base[h*8 + u], so D <= 8.  `save`
-// receives `out` after D-1 updates when nup = D.
-template <int D, int RY, int BY, int NH, bool EDGE, class Save = NoSave>
+// Energy slots are addressed as e[h][u] = base[h*8 + u], so D <= 8.
+template <int D, int RY, int BY, int NH, bool EDGE>
 __device__ __forceinline__ void tv_run_from_zero(const float2* img, float (&out)[RY][2], int nup,
                                                  TileLds<BY>& lds, const LanePos& lp,
                                                  const EdgeMasks& em, float weight, float tw,
-                                                 float (&acc)[NH * 8], bool accumulate,
-                                                 const Save& save = Save()) {
+                                                 float (&acc)[NH * 8], bool accumulate) {
   float p0[RY][2], p1[RY][2];
 #pragma unroll
   for (int j = 0; j < RY; ++j) {
@@ -223,7 +214,6 @@ __device__ __forceinline__ void tv_run_from_zero(const float2* img, float (&out)
 #pragma unroll
   for (int u = 0; u < D; ++u) {
     if (u < nup) {
-      if (u == D - 1) save(out);
       tv_grad<RY, BY, NH, EDGE>(out, p0, p1, lds, lp, em, weight, tw, accumulate ? &acc[u] : nullptr);
       const bool dn = accumulate && u + 1 < D;
       tv_div<RY, BY, NH>(img, out, p0, p1, lds, lp, dn ? &acc[u + 1] : nullptr);
@@ -321,12 +311,11 @@ __device__ __forceinline__ void tv_div_regs(const float2* img, float (&out)[RY][
 // tv_run_from_zero with one barrier per update (plus one at entry, after
 // which the img rows of the wave below are read).  Same arguments and
 // results, bit for bit.
-template <int D, int RY, int BY, int NH, bool EDGE, class Save = NoSave>
+template <int D, int RY, int BY, int NH, bool EDGE>
 __device__ __forceinline__ void tv_run_x1(const float2* img, float (&out)[RY][2], int nup,
                                           TileLdsX1<BY>& lds, const LanePos& lp,
                                           const EdgeMasks& em, float weight, float tw,
-                                          float (&acc)[NH * 8], bool accumulate,
-                                          const Save& save = Save()) {
+                                          float (&acc)[NH * 8], bool accumulate) {
   float p0[RY][2], p1[RY][2];
 #pragma unroll
   for (int j = 0; j < RY; ++j) {
@@ -350,7 +339,6 @@ __device__ __forceinline__ void tv_run_x1(const float2* img, float (&out)[RY][2]
 #pragma unroll
   for (int u = 0; u < D; ++u) {
     if (u < nup) {
-      if (u == D - 1) save(out);
       tv_grad_regs<RY, NH, EDGE>(out, below, p0, p1, em, weight, tw,
                                  accumulate ? &acc[u] : nullptr);
       const int par = u & 1;

@@ -351,8 +351,10 @@ def admmdenoise_cacti(meas, mask, A, At, projmeth='admm', v0=None, orig=None,
         elif projmeth.lower() == 'gap':
             res = gap_denoise(meas_k, msum_host, A_b, At_b, x0=v0_k, X_orig=orig_k, **args)
         else:
+            # The reference prints this and then reads the unbound result of
+            # the solve (pnp_sci_algo.py:518-529): UnboundLocalError.
             print('Unsupported projection method %s' % projmeth.upper())
-            continue
+            raise UnboundLocalError("local variable 'x_k' referenced before assignment")
         x_k, psnr_k, ssim_k, psnrall_k = res
         if flip:
             x_k = x_k[..., ::-1]

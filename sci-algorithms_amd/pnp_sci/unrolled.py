@@ -96,6 +96,11 @@ class _UnrolledFn(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gx):
+        if ctx.ckpt is None:
+            raise RuntimeError("unrolled_gap_tv: the per-iteration checkpoints were released by "
+                               "the first backward pass; call backward once per forward (a "
+                               "second pass through this graph, e.g. after retain_graph=True, "
+                               "needs a new forward)")
         mask, meas, lambdas, weights = ctx.saved_tensors
         H, W, T, B, K, n, logits, measure = ctx.desc
         L = _lib.lib()

@@ -389,3 +389,23 @@ def test_cacti_flips_odd_even_blocks(direction):
                             else v0[..., T:2 * T]), **args)[0]
     want = k1[..., ::-1] if direction == "updown" else k1
     assert np.array_equal(x1, want.astype(np.float32))
+
+
+def test_borderline_case_fp32_drift():
+    """The 9x7x1030 solve (seed 4, 4 iterations, default eps) has exit tests
+    close to their threshold: the reference's float32 arithmetic (this
+    oracle in float32) and FP64 decide one TV call differently and land
+    1.2e-3 apart.  The GPU test of the same case is held to a multiple of
+    this drift (tests/test_gpu_configs.py)."""
+    from pnp_sci.synthetic import make_problem
+    H, W, T, K = 9, 7, 1030, 4
+    pr = make_problem(H, W, T, seed=4)
+    res = {}
+    for dt in (np.float32, np.float64):
+        m = pr.mask.astype(dt)
+        res[dt] = O.gap_denoise(pr.y.astype(dt), pr.phi_sum.astype(dt), lambda v: O.A_(v, m),
+                                lambda v: O.At_(v, m), iter_max=K, tv_weight=0.1,
+                                show_iqa=False)[0]
+    x32, x64 = res[np.float32].astype(np.float64), res[np.float64]
+    drift = np.linalg.norm(x32 - x64) / np.linalg.norm(x64)
+    assert 1.0e-3 < drift < 1.5e-3, drift

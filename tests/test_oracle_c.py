@@ -46,3 +46,33 @@ def test_c_solver_matches_numpy_oracle(method, kw):
         want = O.admm_denoise(y, ps, A, At, iter_max=7, tv_weight=0.1, show_iqa=False,
                               clip=kw.get("clip", False), gamma=kw.get("gamma", 0.01))[0]
     assert np.max(np.abs(got - want)) < 1e-10
+
+
+@pytest.mark.parametrize("method", ["gap", "admm"])
+def test_c_solver_exit_counts_match_numpy_oracle(method, monkeypatch):
+    """The per-iteration early-stop counts of the C oracle equal those of the
+    NumPy oracle's TV calls (eps loose enough that some frames stop)."""
+    rng = np.random.default_rng(5)
+    H, W, T, K = 20, 26, 6, 6
+    mask = (rng.random((H, W, T)) < 0.5).astype(np.float64)
+    truth = rng.random((H, W, T))
+    y = (truth * mask).sum(2)
+    ps = O.phi_sum(mask)
+    calls = []
+    tv64 = O.tv_chambolle_nd
+
+    def counting(image, weight=0.1, eps=2e-4, n_iter_max=200, _stats=None):
+        st = {}
+        res = tv64(image, weight, eps, n_iter_max, _stats=st)
+        calls.append(st["returned"] < n_iter_max - 1)
+        return res
+    monkeypatch.setattr(O, "tv_chambolle_nd", counting)
+    A = lambda v: O.A_(v, mask)  # noqa: E731
+    At = lambda v: O.At_(v, mask)  # noqa: E731
+    run = O.gap_denoise if method == "gap" else O.admm_denoise
+    run(y, ps, A, At, iter_max=K, tv_weight=0.1, show_iqa=False, _tv_eps=2e-2)
+    want = np.array(calls).reshape(K, T).sum(1)
+    _, ex = oracle_c.gap_tv(y, mask, ps, iters=K, method=method, tv_weight=0.1, eps=2e-2,
+                            exits=True)
+    assert 0 < want.sum() < K * T
+    assert ex.tolist() == want.tolist()
