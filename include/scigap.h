@@ -130,6 +130,21 @@ int scigap_tv_chambolle_depth(const float* image, float* out,
                               int* stop_iter, void* workspace, size_t workspace_bytes,
                               void* stream);
 
+/* The TV of scigap_tv_chambolle for n_iter_max <= 7 without a caller
+ * workspace: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * Its scratch (energy slots, stop array, counter) is a device array of
+ * scigap_tv_noalloc_max_bytes() bytes inside the library, one per GPU, held
+ * until the process ends; shapes needing more (scigap_tv_workspace_bytes())
+ * are rejected with SCIGAP_EINVAL, n_iter_max > 7 with SCIGAP_EUNSUPPORTED.
+ * The call allocates nothing.  Calls on one GPU share that array: calls that
+ * overlap in time on one GPU (different streams, or host threads) race. */
+size_t scigap_tv_noalloc_max_bytes(void);
+int scigap_tv_chambolle_noalloc(const float* image, float* out,
+                                int64_t H, int64_t W, int64_t T,
+                                float weight, float eps, int n_iter_max,
+                                int* stop_iter, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* GAP-TV / ADMM-TV solver                                                */
 /* ---------------------------------------------------------------------- */

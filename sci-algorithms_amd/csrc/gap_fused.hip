@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <mutex>
 
 #include "fused.h"
 
@@ -617,6 +618,77 @@ int scigap_tv_chambolle(const float* image, float* out, int64_t H, int64_t W, in
                         size_t workspace_bytes, void* stream) {
   return scigap_tv_chambolle_depth(image, out, H, W, T, weight, eps, n_iter_max, 0, stop_iter,
                                    workspace, workspace_bytes, stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// TV without a caller workspace (scigap_tv_chambolle_noalloc)
+// ---------------------------------------------------------------------------
+
+namespace scigap {
+namespace {
+
+// The scratch of scigap_tv_chambolle_noalloc: the per-tile energy slots, the
+// stop array (when the caller passes none) and the early-stop counter of the
+// single-launch TV.  Every GPU the library runs on holds one copy of it until
+// the process ends.
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device.  The array's name is a
+// host-side stand-in; only this address may be handed to kernels.
+int scratch_address(void** ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(SCIGAP_EUNSUPPORTED, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(SCIGAP_EHIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return SCIGAP_OK;
+}
+
+}  // namespace
+}  // namespace scigap
+
+extern "C" {
+
+size_t scigap_tv_noalloc_max_bytes(void) { return scigap::kNoAllocScratchBytes; }
+
+int scigap_tv_chambolle_noalloc(const float* image, float* out, int64_t H, int64_t W, int64_t T,
+                                float weight, float eps, int n_iter_max, int* stop_iter,
+                                void* stream) {
+  clear_error();
+  if (int rc = check_shape(H, W, T)) return rc;
+  if (n_iter_max < 1) return fail(SCIGAP_EINVAL, "n_iter_max must be >= 1");
+  if (n_iter_max - 1 > kMaxFused)
+    return fail(SCIGAP_EUNSUPPORTED,
+                "scigap_tv_chambolle_noalloc: n_iter_max %d needs the chunked TV and its "
+                "dual-field workspace; at most %d iterations run without one",
+                n_iter_max, kMaxFused + 1);
+  const size_t need = scigap_tv_workspace_bytes_depth(H, W, T, n_iter_max, 0);
+  if (need == 0) return fail(SCIGAP_EINVAL, "scigap_tv_chambolle_noalloc: bad shape");
+  if (need > kNoAllocScratchBytes)
+    return fail(SCIGAP_EINVAL,
+                "scigap_tv_chambolle_noalloc: %lldx%lldx%lld with n_iter_max %d needs %zu bytes "
+                "of scratch, more than the %zu the library holds; use scigap_tv_chambolle",
+                (long long)H, (long long)W, (long long)T, n_iter_max, need,
+                kNoAllocScratchBytes);
+  void* ws = nullptr;
+  if (int rc = scratch_address(&ws)) return rc;
+  return scigap_tv_chambolle_depth(image, out, H, W, T, weight, eps, n_iter_max, 0, stop_iter, ws,
+                                   kNoAllocScratchBytes, stream);
 }
 
 }  // extern "C"

@@ -112,6 +112,9 @@ SYMBOLS = {
     "scigap_tv_chambolle_depth": (_INT, [_P, _P, _I64, _I64, _I64, ctypes.c_float,
                                          ctypes.c_float, _INT, _INT, _P, _P, ctypes.c_size_t,
                                          _P]),
+    "scigap_tv_noalloc_max_bytes": (ctypes.c_size_t, []),
+    "scigap_tv_chambolle_noalloc": (_INT, [_P, _P, _I64, _I64, _I64, ctypes.c_float,
+                                           ctypes.c_float, _INT, _P, _P]),
     "scigap_default_params": (None, [ctypes.POINTER(Params), _I64, _I64, _I64]),
     "scigap_solve_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(Params)]),
     "scigap_solve": (_INT, [ctypes.POINTER(Params), _P, _P, _P, _P, _P, _P, _P, _P, _P,
