@@ -10,6 +10,14 @@
 //   p   = (p - tau*g) / (1 + (tau/weight)*|g|)
 //   E   = sum d^2 + weight * sum |g|     (per frame; drives the early exit)
 //
+// The engine holds the dual field scaled by -1/tau = -4, q = -4p:
+//   q <- (q + g) / (1 + (tau/weight)*|g|),   out = image - dq/4,
+// with dq the same difference stencil applied to q (dq = -4d).  Scaling by
+// a power of two commutes with rounding, so every q is exactly -4 times the
+// reference's p and out, d^2 = dq^2/16 and the energies are the reference's
+// values bit for bit (outside the subnormal range); the dual step then needs
+// an add where p needs an FMA, and the first step from q = 0 only a multiply.
+//
 // Work decomposition.  A workgroup of BY wavefronts owns a load tile of
 // 128 columns x BY*RY rows of one frame (RY = MR*D rows per lane, D = number
 // of fused updates).  Lane l of wave w holds columns c0+2l, c0+2l+1 of rows
@@ -117,8 +125,8 @@ __device__ __forceinline__ void tv_div(const float2* img, float (&out)[RY][2],
     float left0 = from_prev_lane(p1[j][1]);
     d0[j] = (up0 - p0[j][0]) + (left0 - p1[j][0]);
     d1[j] = (up1 - p0[j][1]) + (p1[j][0] - p1[j][1]);
-    out[j][0] = im[j].x + d0[j];
-    out[j][1] = im[j].y + d1[j];
+    out[j][0] = fmaf(-0.25f, d0[j], im[j].x);
+    out[j][1] = fmaf(-0.25f, d1[j], im[j].y);
   }
   float s[NH] = {};
 #pragma unroll
@@ -130,18 +138,18 @@ __device__ __forceinline__ void tv_div(const float2* img, float (&out)[RY][2],
   if (dacc)
 #pragma unroll
     for (int h = 0; h < NH; ++h) {
-      dacc[h * 8] += s[h];
+      dacc[h * 8] = fmaf(0.0625f, s[h], dacc[h * 8]);  // sum dq^2 / 16 = sum d^2
       compute_here(dacc[h * 8]);
     }
 }
 
-// One dual update p <- (p - tau g)/(1 + tw |g|) from the current `out`;
-// adds weight*sum|g| of each strip half h to gacc[h*8] when gacc is non-null.
-template <int RY, int BY, int NH, bool EDGE>
+// One dual update q <- (q + g)/(1 + tw |g|) from the current `out` (ZERO:
+// q = 0 before it, the first update); adds weight*sum|g| of each strip half h
+// to gacc[h*8] when gacc is non-null.
+template <int RY, int BY, int NH, bool EDGE, bool ZERO = false>
 __device__ __forceinline__ void tv_grad(const float (&out)[RY][2], float (&p0)[RY][2],
                                         float (&p1)[RY][2], TileLds<BY>& lds, const LanePos& lp,
                                         const EdgeMasks& em, float weight, float tw, float* gacc) {
-  constexpr float tau = 0.25f;
   float2* xo = reinterpret_cast<float2*>(lds.xo);
   xo[lp.w * 64 + lp.lane] = make_float2(out[0][0], out[0][1]);
   __syncthreads();
@@ -170,10 +178,10 @@ __device__ __forceinline__ void tv_grad(const float (&out)[RY][2], float (&p0)[R
     s[h] += n0 + n1;
     float i0 = fast_rcp(fmaf(tw, n0, 1.f));
     float i1 = fast_rcp(fmaf(tw, n1, 1.f));
-    p0[j][0] = fmaf(-tau, g00, p0[j][0]) * i0;
-    p1[j][0] = fmaf(-tau, g10, p1[j][0]) * i0;
-    p0[j][1] = fmaf(-tau, g01, p0[j][1]) * i1;
-    p1[j][1] = fmaf(-tau, g11, p1[j][1]) * i1;
+    p0[j][0] = (ZERO ? g00 : p0[j][0] + g00) * i0;
+    p1[j][0] = (ZERO ? g10 : p1[j][0] + g10) * i0;
+    p0[j][1] = (ZERO ? g01 : p0[j][1] + g01) * i1;
+    p1[j][1] = (ZERO ? g11 : p1[j][1] + g11) * i1;
     // Materialise the new dual values here; otherwise the compiler folds the
     // multiply by 1/den into the next divergence and keeps both factors live.
     asm("" : "+v"(p0[j][0]), "+v"(p1[j][0]), "+v"(p0[j][1]), "+v"(p1[j][1]));
@@ -214,7 +222,12 @@ __device__ __forceinline__ void tv_run_from_zero(const float2* img, float (&out)
 #pragma unroll
   for (int u = 0; u < D; ++u) {
     if (u < nup) {
-      tv_grad<RY, BY, NH, EDGE>(out, p0, p1, lds, lp, em, weight, tw, accumulate ? &acc[u] : nullptr);
+      if (u == 0)
+        tv_grad<RY, BY, NH, EDGE, true>(out, p0, p1, lds, lp, em, weight, tw,
+                                        accumulate ? &acc[u] : nullptr);
+      else
+        tv_grad<RY, BY, NH, EDGE>(out, p0, p1, lds, lp, em, weight, tw,
+                                  accumulate ? &acc[u] : nullptr);
       const bool dn = accumulate && u + 1 < D;
       tv_div<RY, BY, NH>(img, out, p0, p1, lds, lp, dn ? &acc[u + 1] : nullptr);
     }
@@ -237,12 +250,11 @@ struct TileLdsX1 {
 
 // Dual update from `out` with the next wave's first `out` row in `below`
 // (no LDS traffic); adds weight*sum|g| per strip half to gacc[h*8].
-template <int RY, int NH, bool EDGE>
+template <int RY, int NH, bool EDGE, bool ZERO = false>
 __device__ __forceinline__ void tv_grad_regs(const float (&out)[RY][2], float2 below,
                                              float (&p0)[RY][2], float (&p1)[RY][2],
                                              const EdgeMasks& em, float weight, float tw,
                                              float* gacc) {
-  constexpr float tau = 0.25f;
   float s[NH] = {};
 #pragma unroll
   for (int j = 0; j < RY; ++j) {
@@ -266,10 +278,10 @@ __device__ __forceinline__ void tv_grad_regs(const float (&out)[RY][2], float2 b
     s[h] += n0 + n1;
     float i0 = fast_rcp(fmaf(tw, n0, 1.f));
     float i1 = fast_rcp(fmaf(tw, n1, 1.f));
-    p0[j][0] = fmaf(-tau, g00, p0[j][0]) * i0;
-    p1[j][0] = fmaf(-tau, g10, p1[j][0]) * i0;
-    p0[j][1] = fmaf(-tau, g01, p0[j][1]) * i1;
-    p1[j][1] = fmaf(-tau, g11, p1[j][1]) * i1;
+    p0[j][0] = (ZERO ? g00 : p0[j][0] + g00) * i0;
+    p1[j][0] = (ZERO ? g10 : p1[j][0] + g10) * i0;
+    p0[j][1] = (ZERO ? g01 : p0[j][1] + g01) * i1;
+    p1[j][1] = (ZERO ? g11 : p1[j][1] + g11) * i1;
     asm("" : "+v"(p0[j][0]), "+v"(p1[j][0]), "+v"(p0[j][1]), "+v"(p1[j][1]));
   }
   if (gacc)
@@ -294,8 +306,8 @@ __device__ __forceinline__ void tv_div_regs(const float2* img, float (&out)[RY][
     float d0 = (up0 - p0[j][0]) + (left0 - p1[j][0]);
     float d1 = (up1 - p0[j][1]) + (p1[j][0] - p1[j][1]);
     const float2 im = img[j * 64];
-    out[j][0] = im.x + d0;
-    out[j][1] = im.y + d1;
+    out[j][0] = fmaf(-0.25f, d0, im.x);
+    out[j][1] = fmaf(-0.25f, d1, im.y);
     const int h = (NH == 2 && j >= RY / 2) ? 1 : 0;
     s[h] = fmaf(d0, d0, s[h]);
     s[h] = fmaf(d1, d1, s[h]);
@@ -303,7 +315,7 @@ __device__ __forceinline__ void tv_div_regs(const float2* img, float (&out)[RY][
   if (dacc)
 #pragma unroll
     for (int h = 0; h < NH; ++h) {
-      dacc[h * 8] += s[h];
+      dacc[h * 8] = fmaf(0.0625f, s[h], dacc[h * 8]);  // sum dq^2 / 16 = sum d^2
       compute_here(dacc[h * 8]);
     }
 }
@@ -339,8 +351,12 @@ __device__ __forceinline__ void tv_run_x1(const float2* img, float (&out)[RY][2]
 #pragma unroll
   for (int u = 0; u < D; ++u) {
     if (u < nup) {
-      tv_grad_regs<RY, NH, EDGE>(out, below, p0, p1, em, weight, tw,
-                                 accumulate ? &acc[u] : nullptr);
+      if (u == 0)
+        tv_grad_regs<RY, NH, EDGE, true>(out, below, p0, p1, em, weight, tw,
+                                         accumulate ? &acc[u] : nullptr);
+      else
+        tv_grad_regs<RY, NH, EDGE>(out, below, p0, p1, em, weight, tw,
+                                   accumulate ? &acc[u] : nullptr);
       const int par = u & 1;
       lds.last0[par][me] = make_float2(p0[RY - 1][0], p0[RY - 1][1]);
       lds.first0[par][me] = make_float2(p0[0][0], p0[0][1]);
@@ -358,7 +374,7 @@ __device__ __forceinline__ void tv_run_x1(const float2* img, float (&out)[RY][2]
       const float left0 = from_prev_lane(bp1.y);
       const float d0 = (p0[RY - 1][0] - bp0.x) + (left0 - bp1.x);
       const float d1 = (p0[RY - 1][1] - bp0.y) + (bp1.x - bp1.y);
-      below = has_below ? make_float2(bimg.x + d0, bimg.y + d1)
+      below = has_below ? make_float2(fmaf(-0.25f, d0, bimg.x), fmaf(-0.25f, d1, bimg.y))
                         : make_float2(out[RY - 1][0], out[RY - 1][1]);
     }
   }
