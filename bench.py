@@ -358,7 +358,10 @@ def batched(args):
     torch.cuda.synchronize()
     t_torch = time.perf_counter() - t0
 
-    plan = BatchPlan(dev, iters=K, tv_weight=tvw, use_graph=True)
+    from pnp_sci import tune
+    log("autotuning the batched plan")
+    best, cand_times = tune.autotune_batch(dev, iters=K, tv_weight=tvw)
+    plan = BatchPlan(dev, iters=K, tv_weight=tvw, use_graph=True, tune=True)
     t_batch = _time_plan(plan)
 
     def mean_psnr(xs):
@@ -403,7 +406,11 @@ def batched(args):
         "size_counts": sizes,
         "batched": {"value": pfi / t_batch, "ms": t_batch * 1e3,
                     "algorithmic_GBps": bytes_it * K / t_batch / 1e9,
-                    "mean_psnr_db": ps_batch, "mode": "persistent tile queue, hipGraph replay"},
+                    "mean_psnr_db": ps_batch, "mode": "persistent tile queue, hipGraph replay",
+                    "tuned_options": plan.options_used,
+                    "tuning_ms": {"tile_rows=%d grid=%d order=%d" % (
+                        dict(k)["tile_rows"], dict(k)["grid"], dict(k)["order"]): v * 1e3
+                        for k, v in cand_times.items()}},
         "one_at_a_time": {"value": pfi / t_seq, "ms": t_seq * 1e3,
                           "mean_psnr_db": ps_seq,
                           "mode": "scigap_solve per measurement, same fused kernels"},

@@ -40,13 +40,30 @@ class BatchPlan:
     ``problems``: list of (y (H,W), phi (T,H,W), phi_sum (H,W), x0 (T,H,W) or
     None) float32 CUDA tensors.  Outputs are allocated here (``self.x``).
     Solver options are shared by all solves.
+
+    ``tune=True`` times the plan options on this batch first
+    (``pnp_sci.tune.autotune_batch``) and uses the fastest; with the default
+    options (tile_rows = grid = order = 0) a choice remembered for the same
+    batch signature is used without tuning again.  ``self.options_used``
+    holds the options of the plan.
     """
 
     def __init__(self, problems, *, method="gap", iters=50, _lambda=1.0, accelerate=True,
                  gamma=0.01, clip=False, tv_weight=0.1, tv_iter_max=5, tv_eps=2e-4,
-                 use_graph=True, tile_rows=0, grid=0, order=0, stream=None):
+                 use_graph=True, tile_rows=0, grid=0, order=0, tune=False, stream=None):
         import torch
         L = _lib.lib()
+        if tune or (tile_rows == 0 and grid == 0 and order == 0):
+            from . import tune as _tune
+            solver = dict(method=method, iters=iters, _lambda=_lambda, accelerate=accelerate,
+                          gamma=gamma, clip=clip, tv_weight=tv_weight, tv_iter_max=tv_iter_max,
+                          tv_eps=tv_eps)
+            best = _tune.tuned_batch_options(problems, solver)
+            if best is None and tune:
+                best, _ = _tune.autotune_batch(problems, **solver)
+            if best is not None:
+                tile_rows, grid, order = best["tile_rows"], best["grid"], best["order"]
+        self.options_used = dict(tile_rows=tile_rows, grid=grid, order=order)
         self._L = L
         self._keep = []
         self.x = []

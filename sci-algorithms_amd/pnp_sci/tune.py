@@ -67,17 +67,39 @@ def autotune_tv(image_thw, *, weight=0.1, eps=2e-4, n_iter_max=30):
     return _result(r)
 
 
+# Batch choices of this process: {batch signature: options}.
+_batch_cache = {}
+
+
+def batch_signature(problems, solver):
+    """Key of a tuned batch: the shapes in submission order and the solver
+    options that change the kernels' work (method, TV depth, iterations)."""
+    shapes = tuple(tuple(int(v) for v in p[1].shape) for p in problems)
+    # BatchPlan's defaults for options the caller left out
+    defaults = (("method", "gap"), ("accelerate", True), ("tv_iter_max", 5), ("iters", 50))
+    return shapes, tuple((k, solver.get(k, d)) for k, d in defaults)
+
+
+def tuned_batch_options(problems, solver):
+    """Options remembered for this batch by autotune_batch, or None."""
+    return _batch_cache.get(batch_signature(problems, solver))
+
+
 def autotune_batch(problems, *, reps=2, candidates=None, **solver):
-    """Time BatchPlan options on the batch itself; returns (best options,
-    {options: seconds per run}).  ``problems`` as for BatchPlan."""
+    """Time BatchPlan options (tile height of the fused kernel, persistent
+    grid, queue order) on the batch itself, remember the fastest for this
+    batch signature (used by BatchPlan(..., tune=True) and by later plans of
+    the same batch with default options), and return (best options,
+    {options: seconds per run}).  ``problems`` as for BatchPlan.  The choice
+    never changes a result (the same tile body runs in every candidate)."""
     import torch
     from .batch import BatchPlan
     if candidates is None:
         candidates = [dict(tile_rows=r, grid=g, order=o)
-                      for r in (12, 8, 16) for g in (256, 512) for o in (0,)]
+                      for r in (8, 12, 16) for g in (0, 256) for o in (0, 1)]
     times = {}
     for c in candidates:
-        plan = BatchPlan(problems, use_graph=True, **c, **solver)
+        plan = BatchPlan(problems, use_graph=True, tune=False, **c, **solver)
         plan.run()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -87,9 +109,11 @@ def autotune_batch(problems, *, reps=2, candidates=None, **solver):
         times[tuple(sorted(c.items()))] = (time.perf_counter() - t0) / reps
         plan.close()
     best = min(times, key=times.get)
+    _batch_cache[batch_signature(problems, solver)] = dict(best)
     return dict(best), times
 
 
 def clear():
     """Forget every tuned choice of this process."""
+    _batch_cache.clear()
     _lib.check(_lib.lib().scigap_tune_clear())

@@ -126,3 +126,34 @@ def test_batch_rejects_unsupported(gpu):
         BatchPlan(dev, iters=4, tv_iter_max=12)   # deep TV is not batched
     with pytest.raises(ValueError):
         BatchPlan([], iters=4)
+
+
+def test_batch_autotune_picks_and_remembers(gpu):
+    """BatchPlan(tune=True) times the plan options on the batch, uses the
+    fastest, and a later plan of the same batch with default options reuses
+    the choice; the tuned plan's results equal the default plan's."""
+    from pnp_sci import tune
+    from pnp_sci.batch import BatchPlan
+    probs = _problems(seed=3)
+    dev = [_device(p) for p in probs]
+    tune.clear()
+    base = BatchPlan(dev, iters=6, tv_weight=0.1, use_graph=False, tile_rows=8, grid=0, order=1)
+    want = [x.cpu().numpy() for x in base.run()]
+    base.close()
+    cands = [dict(tile_rows=r, grid=g, order=o) for r in (8, 16) for g in (0, 128) for o in (0,)]
+    best, times = tune.autotune_batch(dev, candidates=cands, iters=6, tv_weight=0.1)
+    assert best in cands and len(times) == len(cands)
+    plan = BatchPlan(dev, iters=6, tv_weight=0.1, use_graph=True, tune=True)
+    assert plan.options_used == best          # remembered: no second tuning run
+    got = [x.cpu().numpy() for x in plan.run()]
+    plan.close()
+    again = BatchPlan(dev, iters=6, tv_weight=0.1, use_graph=False)
+    assert again.options_used == best
+    again.close()
+    for g, w in zip(got, want):
+        assert rel(g, w) <= 1e-6
+    tune.clear()
+    assert tune.tuned_batch_options(dev, dict(method="gap", iters=6, _lambda=1.0,
+                                              accelerate=True, gamma=0.01, clip=False,
+                                              tv_weight=0.1, tv_iter_max=5,
+                                              tv_eps=2e-4)) is None
