@@ -67,6 +67,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float* p, int
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, unsigned(N) * 4u,
                                            0x00020000);
 }
+// Two floats per lane in one buffer_load_dwordx2: the LLVM intrinsic itself
+// (clang's 64-bit raw buffer-load builtin lowers to a single dword load on
+// this toolchain).  Compiler-managed like the builtins: no inline asm.
+typedef float buffer_f2 __attribute__((ext_vector_type(2)));
+extern "C" __device__ buffer_f2 raw_buffer_load_f2(__amdgpu_buffer_rsrc_t rsrc, int voffset,
+                                                   int soffset, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f32");
+
 __device__ __forceinline__ float ld1(const float* p, int N, unsigned o) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(plane_rsrc(p, N), o * 4u,
                                                                         0, 0));
@@ -77,23 +85,21 @@ __device__ __forceinline__ void st1(float* p, int N, unsigned o, float v) {
 
 // Load an RY x 2 register block of one plane (zero outside the image).
 // `off` is the plane offset of register (0,0).  INSIDE tiles lie entirely in
-// the image (and W is even): one 8-byte buffer load per row (two merged dword loads), the row step in
-// the scalar offset.  Border tiles send pixels outside the image to an offset
-// past the end of the buffer, which reads as 0.
+// the image (and W is even): one 8-byte buffer load per row (a wave reads 512
+// contiguous bytes per instruction), the row step in the scalar offset.
+// Border tiles send pixels outside the image to an offset past the end of
+// the buffer, which reads as 0.
 template <int RY, bool INSIDE>
 __device__ __forceinline__ void load_plane(const float* __restrict__ src, float (&v)[RY][2],
                                            const LanePos& lp, int off, int H, int W) {
   const __amdgpu_buffer_rsrc_t rs = plane_rsrc(src, H * W);
   if constexpr (INSIDE) {
-    const unsigned vo = unsigned(off) * 4u;
+    const int vo = off * 4;
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
-      // Two dword loads at adjacent offsets: the backend merges them into one
-      // buffer_load_dwordx2 (the 64-bit builtin lowers to a single dword load
-      // on this toolchain, so it is not used).
-      v[j][0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, vo, j * W * 4, 0));
-      v[j][1] = __builtin_bit_cast(float,
-                                   __builtin_amdgcn_raw_buffer_load_b32(rs, vo + 4u, j * W * 4, 0));
+      const buffer_f2 f = raw_buffer_load_f2(rs, vo, j * W * 4, 0);
+      v[j][0] = f.x;
+      v[j][1] = f.y;
     }
   } else {
 #pragma unroll
