@@ -199,12 +199,19 @@ __device__ __forceinline__ void tv_grad(const float (&out)[RY][2], float (&p0)[R
 // (row j at img[j*64]).  On return `out` holds the image after nup updates and
 // acc.e[h][u] this lane's partial of E_u over half h of its strip.  When
 // `accumulate` is false the energies are not formed (recomputation passes).
-// Energy slots are addressed as e[h][u] = base[h*8 + u], so D <= 8.
-template <int D, int RY, int BY, int NH, bool EDGE>
+// Energy slots are addressed as e[h][u] = base[h*8 + u], so D <= 8.  `hook`
+// is called with u after the dual update u (the caller spreads its memory
+// requests over the updates with it).
+struct NoHook {
+  __device__ void operator()(int) const {}
+};
+
+template <int D, int RY, int BY, int NH, bool EDGE, class Hook = NoHook>
 __device__ __forceinline__ void tv_run_from_zero(const float2* img, float (&out)[RY][2], int nup,
                                                  TileLds<BY>& lds, const LanePos& lp,
                                                  const EdgeMasks& em, float weight, float tw,
-                                                 float (&acc)[NH * 8], bool accumulate) {
+                                                 float (&acc)[NH * 8], bool accumulate,
+                                                 const Hook& hook = Hook()) {
   float p0[RY][2], p1[RY][2];
 #pragma unroll
   for (int j = 0; j < RY; ++j) {
@@ -228,6 +235,7 @@ __device__ __forceinline__ void tv_run_from_zero(const float2* img, float (&out)
       else
         tv_grad<RY, BY, NH, EDGE>(out, p0, p1, lds, lp, em, weight, tw,
                                   accumulate ? &acc[u] : nullptr);
+      hook(u);
       const bool dn = accumulate && u + 1 < D;
       tv_div<RY, BY, NH>(img, out, p0, p1, lds, lp, dn ? &acc[u + 1] : nullptr);
     }
