@@ -794,16 +794,33 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
     }
     float* xo = a.x_out + size_t(t) * N;
     float xold[RY][2] = {};
-    if constexpr (!kTvOnly) load_plane<RY, INSIDE>(xo, xold, lp, off0, H, W);
     const int tn = next_stopped(t + 1);
-    if (tn < T) {
-      if constexpr (!kTvOnly) load_plane<RY, INSIDE>(a.phi + size_t(tn) * N, nph, lp, off0, H, W);
-      if (!from_y) load_plane<RY, INSIDE>(a.x_in + size_t(tn) * N, nx, lp, off0, H, W);
-    }
+    // This frame's old output and the next stopped frame's Phi and x are
+    // requested row by row over the frame's dual updates (as in the main
+    // pass), so that the waves' requests do not all meet the first barrier.
+    const float* nphi = a.phi + size_t(tn < T ? tn : t) * N;
+    const float* nxin = a.x_in + size_t(tn < T ? tn : t) * N;
+    const int spread = nup < 3 ? nup : 3;
+    auto requests = [&](int u) {
+      constexpr int total = 3 * RY;
+      const int lo = (u * total) / spread, hi = ((u + 1) * total) / spread;
+#pragma unroll
+      for (int r = 0; r < total; ++r) {
+        if (r < lo || r >= hi) continue;
+        if (r < RY) {
+          if constexpr (!kTvOnly) load_row<RY, INSIDE>(xo, xold, lp, off0, H, W, r);
+        } else if (r < 2 * RY) {
+          if constexpr (!kTvOnly)
+            if (tn < T) load_row<RY, INSIDE>(nphi, nph, lp, off0, H, W, r - RY);
+        } else if (!from_y && tn < T) {
+          load_row<RY, INSIDE>(nxin, nx, lp, off0, H, W, r - 2 * RY);
+        }
+      }
+    };
     float out[RY][2];
     float acc[NH * 8];
     tv_run_from_zero<D, RY, BY, NH, !INSIDE>(simg, out, nup, lds, lp, em, a.weight, a.tw, acc,
-                                             false);
+                                             false, requests);
     const float* trt = a.truth ? a.truth + size_t(t) * N : nullptr;
     float* bt = a.b ? a.b + size_t(t) * N : nullptr;
 #pragma unroll
