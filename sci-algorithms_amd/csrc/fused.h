@@ -823,30 +823,38 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
                                              false, requests);
     const float* trt = a.truth ? a.truth + size_t(t) * N : nullptr;
     float* bt = a.b ? a.b + size_t(t) * N : nullptr;
+    float bv[RY][2];
 #pragma unroll
     for (int j = 0; j < RY; ++j)
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
-        if (!tp.owns(j, k)) continue;
-        const unsigned o = unsigned(off0 + j * W + k);
         float xn = out[j][k];
         if constexpr (MODE == ADMM) {
           if (a.clip) xn = fminf(fmaxf(xn, 0.f), 1.f);
-          if (a.write_b) {
-            const float2 im = simg[j * 64];  // this lane's own slot
-            bt[o] = xn - (k ? im.y : im.x);
-          }
-          dyb[j][k] = fmaf(2.f * ph[j][k], xn - xold[j][k], dyb[j][k]);
+          const float2 im = simg[j * 64];  // this lane's own slot
+          bv[j][k] = xn - (k ? im.y : im.x);
+          if (tp.owns(j, k)) dyb[j][k] = fmaf(2.f * ph[j][k], xn - xold[j][k], dyb[j][k]);
         } else if constexpr (!kTvOnly) {
-          if (trt) {  // replace the main pass's error of this pixel
-            float e = trt[o] - xn, eo = trt[o] - xold[j][k];
-            err = fmaf(e, e, err);
-            err = fmaf(-eo, eo, err);
+          if (tp.owns(j, k)) {
+            if (trt) {  // replace the main pass's error of this pixel
+              const unsigned o = unsigned(off0 + j * W + k);
+              float e = trt[o] - xn, eo = trt[o] - xold[j][k];
+              err = fmaf(e, e, err);
+              err = fmaf(-eo, eo, err);
+            }
+            dyb[j][k] = fmaf(ph[j][k], xn - xold[j][k], dyb[j][k]);
           }
-          dyb[j][k] = fmaf(ph[j][k], xn - xold[j][k], dyb[j][k]);
         }
-        xo[o] = xn;
+        out[j][k] = xn;
       }
+    // Owned pixels only, one 8-byte store per row where both columns are owned.
+    if (tp.own_c[0]) {
+      unsigned base = unsigned(off0) * 4u;
+      compute_here(base);
+      store_owned<D, BY, MR, INSIDE>(xo, N, W, tp, base, out);
+      if constexpr (MODE == ADMM)
+        if (a.write_b) store_owned<D, BY, MR, INSIDE>(bt, N, W, tp, base, bv);
+    }
     t = tn;
   }
   if constexpr (!kTvOnly) {
