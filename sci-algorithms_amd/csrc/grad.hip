@@ -42,6 +42,7 @@ namespace scigap {
 namespace {
 
 constexpr int BX = 64, BYR = 4;  // 256-thread blocks of the per-pixel kernels
+constexpr int kGroup = 4;        // frames whose loads a per-pixel kernel issues together
 constexpr float kTau = 0.25f;
 
 // Block reduction of one float into a double accumulator (scaled).
@@ -58,21 +59,41 @@ __device__ __forceinline__ void block_atomic(float v, double* dst, double scale)
 }
 
 // Reverse of the projection v = x + lambda*r*Phi, r = (y1' - yb)/S,
-// y1' = y1 + y - yb, yb = sum_t Phi_t x_t (per pixel, all frames).
+// y1' = y1 + y - yb, yb = sum_t Phi_t x_t (per pixel, all frames), given
+// vbar = dL/dv.  The per-pixel part only: rbar = lambda sum_t vbar_t Phi_t,
+// lambar += sum vbar r Phi, y1bar' = y1bar_next + rbar/S, ybb = -rbar/S - y1bar',
+// ybar += y1bar', Sbar = -rbar r / S.  The frame part xbar_t = vbar_t + ybb Phi_t
+// is never stored: the next reverse TV step forms it on load, and the mask
+// gradient (k_ub_phibar) takes ybb and Sbar from the planes written here.
 __global__ void __launch_bounds__(256) k_ub_proj(
-    const float* __restrict__ vbar, const float* __restrict__ x, const float* __restrict__ phi,
-    const float* __restrict__ S, const float* __restrict__ r, float lam,
-    const float* __restrict__ y1bar_ext, float* __restrict__ xbar_out,
-    float* __restrict__ y1bar_out, float* __restrict__ ybar, float* __restrict__ phibar,
-    double* lambar, int64_t N, int T) {
+    const float* __restrict__ vbar, const float* __restrict__ phi, const float* __restrict__ S,
+    const float* __restrict__ r, float lam, const float* __restrict__ y1bar_ext,
+    float* __restrict__ y1bar_out, float* __restrict__ ybar, float* __restrict__ ybb_out,
+    float* __restrict__ sbar_out, double* lambar, int64_t N, int T) {
   float lsum = 0.f;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
        i += int64_t(gridDim.x) * blockDim.x) {
     const float ri = r[i], si = S[i];
     double rsum = 0.0;
     float fsum = 0.f;
-    for (int t = 0; t < T; ++t) {
-      const float vb = vbar[t * N + i], f = phi[t * N + i];
+    // Frames in groups of kGroup with all loads of a group issued together
+    // (one memory latency per group); the sums keep frame order.
+    int t = 0;
+    for (; t + kGroup <= T; t += kGroup) {
+      float vb[kGroup], f[kGroup];
+#pragma unroll
+      for (int g = 0; g < kGroup; ++g) {
+        vb[g] = vbar[size_t(t + g) * N + i];
+        f[g] = phi[size_t(t + g) * N + i];
+      }
+#pragma unroll
+      for (int g = 0; g < kGroup; ++g) {
+        rsum = acc_prod(vb[g], f[g], rsum);
+        fsum += f[g];
+      }
+    }
+    for (; t < T; ++t) {
+      const float vb = vbar[size_t(t) * N + i], f = phi[size_t(t) * N + i];
       rsum = acc_prod(vb, f, rsum);
       fsum += f;
     }
@@ -80,24 +101,35 @@ __global__ void __launch_bounds__(256) k_ub_proj(
     lsum = fmaf(rbar, ri, lsum);  // dL/dlambda = sum_t vbar*r*Phi
     rbar *= lam;
     const float y1tot = (y1bar_ext ? y1bar_ext[i] : 0.f) + rbar / si;
-    const float ybb = -rbar / si - y1tot;
-    // S = sum_t Phi_t, except where that sum is 0 and S is the constant 1.
-    const float sbar = fsum != 0.f ? -rbar * ri / si : 0.f;
     y1bar_out[i] = y1tot;
     ybar[i] += y1tot;
-    const float lr = lam * ri;
-    for (int t = 0; t < T; ++t) {
-      const size_t o = size_t(t) * N + i;
-      const float vb = vbar[o], f = phi[o], xv = x[o];
-      xbar_out[o] = fmaf(ybb, f, vb);
-      if (phibar) phibar[o] += fmaf(lr, vb, fmaf(ybb, xv, sbar));
-    }
+    ybb_out[i] = -rbar / si - y1tot;
+    // S = sum_t Phi_t, except where that sum is 0 and S is the constant 1.
+    if (sbar_out) sbar_out[i] = fsum != 0.f ? -rbar * ri / si : 0.f;
   }
   block_atomic(lsum, lambar, 1.0);
 }
 
-// Reverse of x_0 = Phi*y: phibar += xbar_0 * y, ybar += sum_t xbar_0 * Phi.
-__global__ void __launch_bounds__(256) k_ub_init(const float* __restrict__ xbar,
+// Mask gradient of one projection: phibar_t += lambda r vbar_t + ybb x_t + Sbar.
+__global__ void __launch_bounds__(256) k_ub_phibar(
+    const float* __restrict__ vbar, const float* __restrict__ x, const float* __restrict__ r,
+    float lam, const float* __restrict__ ybb, const float* __restrict__ sbar,
+    float* __restrict__ phibar, int64_t N, int T) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const float lr = lam * r[i], yb = ybb[i], sb = sbar[i];
+#pragma unroll kGroup
+    for (int t = 0; t < T; ++t) {
+      const size_t o = size_t(t) * N + i;
+      phibar[o] += fmaf(lr, vbar[o], fmaf(yb, x[o], sb));
+    }
+  }
+}
+
+// Reverse of x_0 = Phi*y: phibar += xbar_0 * y, ybar += sum_t xbar_0 * Phi,
+// with xbar_0 = vbar + ybb Phi formed here (ybb null: xbar_0 = vbar).
+__global__ void __launch_bounds__(256) k_ub_init(const float* __restrict__ vbar,
+                                                 const float* __restrict__ ybb,
                                                  const float* __restrict__ phi,
                                                  const float* __restrict__ y,
                                                  float* __restrict__ ybar,
@@ -105,11 +137,14 @@ __global__ void __launch_bounds__(256) k_ub_init(const float* __restrict__ xbar,
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < N;
        i += int64_t(gridDim.x) * blockDim.x) {
     double acc = 0.0;
-    const float yv = y[i];
+    const float yv = y[i], yb = ybb ? ybb[i] : 0.f;
+#pragma unroll kGroup
     for (int t = 0; t < T; ++t) {
       const size_t o = size_t(t) * N + i;
-      acc = acc_prod(xbar[o], phi[o], acc);
-      if (phibar) phibar[o] = fmaf(xbar[o], yv, phibar[o]);
+      const float f = phi[o];
+      const float xb = ybb ? fmaf(yb, f, vbar[o]) : vbar[o];
+      acc = acc_prod(xb, f, acc);
+      if (phibar) phibar[o] = fmaf(xb, yv, phibar[o]);
     }
     ybar[i] += float(acc);
   }
@@ -164,14 +199,14 @@ __global__ void k_measure_bwd(const float* __restrict__ ybar, const float* __res
 }  // namespace
 
 int tv_backward_tiles(int D, const float* x, const float* phi, const float* r, float lam,
-                      const float* xbar, float* vbar, int64_t H, int64_t W, int64_t T, float w,
-                      double* wbar, hipStream_t s);
+                      const float* xbar, const float* ybb, float* vbar, int64_t H, int64_t W,
+                      int64_t T, float w, double* wbar, hipStream_t s);
 
 size_t unrolled_backward_workspace(int64_t H, int64_t W, int64_t T, int D) {
   (void)D;
   const size_t N = size_t(H) * W, V = N * size_t(T);
-  // vbar, xbar x2, y1bar x2
-  return align256(V * 4) + 2 * align256(V * 4) + 2 * align256(N * 4);
+  // vbar x2, y1bar x2, ybb x2, Sbar
+  return 2 * align256(V * 4) + 5 * align256(N * 4);
 }
 
 // Backward through K unrolled iterations of one sample.
@@ -189,39 +224,48 @@ int unrolled_backward(int64_t H, int64_t W, int64_t T, int K, int D, const float
     off += align256(bytes);
     return reinterpret_cast<float*>(q);
   };
-  float* vbar = take(V * 4);
-  float* xbar[2] = {take(V * 4), take(V * 4)};
+  float* vbar[2] = {take(V * 4), take(V * 4)};
   float* y1bar[2] = {take(N * 4), take(N * 4)};
+  float* ybb[2] = {take(N * 4), take(N * 4)};
+  float* sbar = take(N * 4);
 
   k_fill_f32<<<grid1(N), 256, 0, s>>>(grad_y, N, 0.f);
   SCIGAP_LAUNCH_CHECK("grad fill");
-  const float* xb_in = grad_x;  // adjoint of x_{k+1}
+  // Adjoint of x_{k+1}: grad_x for k = K-1, else vbar_{k+1} + ybb_{k+1} Phi
+  // (formed on load by the reverse TV step and by k_ub_init).
+  const float* vb_in = grad_x;
+  const float* ybb_in = nullptr;
   const float* y1b_in = nullptr;
-  int xsel = 0, ysel = 0;
+  int sel = 0;
   for (int k = K - 1; k >= 0; --k) {
     const float lam = lambdas[k], w = weights[k];
     const float* xk = ckpt_x + size_t(k) * xstride;
     const float* r = ckpt_r + size_t(k) * rstride;  // residual the forward used
     {
       ProfScope ps("unrolled_bwd_tv", s);
-      if (int rc = tv_backward_tiles(D, xk, phi, r, lam, xb_in, vbar, H, W, T, w, grad_weight + k,
-                                     s))
+      if (int rc = tv_backward_tiles(D, xk, phi, r, lam, vb_in, ybb_in, vbar[sel], H, W, T, w,
+                                     grad_weight + k, s))
         return rc;
     }
     {
       ProfScope ps("unrolled_bwd_proj", s);
-      k_ub_proj<<<grid1(N), 256, 0, s>>>(vbar, xk, phi, S, r, lam, y1b_in, xbar[xsel],
-                                         y1bar[ysel], grad_y, grad_phi, grad_lambda + k, N,
-                                         int(T));
+      k_ub_proj<<<grid1(N), 256, 0, s>>>(vbar[sel], phi, S, r, lam, y1b_in, y1bar[sel], grad_y,
+                                         ybb[sel], grad_phi ? sbar : nullptr, grad_lambda + k,
+                                         N, int(T));
+      SCIGAP_LAUNCH_CHECK("unrolled projection backward");
+      if (grad_phi) {
+        k_ub_phibar<<<grid1(N), 256, 0, s>>>(vbar[sel], xk, r, lam, ybb[sel], sbar, grad_phi,
+                                             N, int(T));
+        SCIGAP_LAUNCH_CHECK("unrolled mask gradient");
+      }
     }
-    SCIGAP_LAUNCH_CHECK("unrolled projection backward");
-    xb_in = xbar[xsel];
-    y1b_in = y1bar[ysel];
-    xsel ^= 1;
-    ysel ^= 1;
+    vb_in = vbar[sel];
+    ybb_in = ybb[sel];
+    y1b_in = y1bar[sel];
+    sel ^= 1;
   }
   // x_0 = Phi * y
-  k_ub_init<<<grid1(N), 256, 0, s>>>(xb_in, phi, y, grad_y, grad_phi, N, int(T));
+  k_ub_init<<<grid1(N), 256, 0, s>>>(vb_in, ybb_in, phi, y, grad_y, grad_phi, N, int(T));
   SCIGAP_LAUNCH_CHECK("unrolled init backward");
   return SCIGAP_OK;
 }

@@ -13,7 +13,7 @@
 //     gbar = -tau abar + (nbar / n_u) g_u   (0 where n_u = 0),
 //     obar = G^T gbar,  vbar += obar,  pbar_u = abar + G obar.
 //
-// Nothing but x, Phi, r, xbar and vbar touches memory: the dual field p_u that step u
+// Nothing but x, Phi, r, xbar (and ybb) and vbar touches memory: the dual field p_u that step u
 // needs is recomputed in registers (the same arithmetic as the forward
 // engine), from the forward state after D/2 updates, kept in registers, for
 // u >= D/2 and from p_0 = 0 otherwise.  Layout: a workgroup of 16 waves holds a
@@ -24,7 +24,8 @@
 // at wave boundaries are exchanged through LDS (one barrier per exchange,
 // two alternating buffers).  Each stencil makes one more ring of the load
 // tile stale, 2D rings at most on every side, so the workgroup owns the
-// 128-4D x 64-4D interior.
+// 128-4D x 64-4D interior, and up to the image edge on a side where the tile
+// meets it (nothing is stale there: the masks are exact).
 
 #include "fused.h"
 
@@ -54,10 +55,12 @@ struct BwdArgs {
   const float* phi;   // (T,H,W) mask
   const float* r;     // (H,W) residual r_k the forward used
   float lam;          // lambda_k
-  const float* xbar;  // (T,H,W) adjoint of the TV output
+  const float* xbar;  // (T,H,W) adjoint of the TV output, or with ybb: vbar of the next
+                      // outer iteration, xbar = xbar + ybb * Phi formed on load
+  const float* ybb;   // (H,W) or null (see xbar)
   float* vbar;        // (T,H,W) adjoint of the TV input
   double* wbar;       // adjoint of the TV weight (accumulated)
-  int H, W, tiles_x;
+  int H, W, tiles_x, tiles_y;
   float tw;           // tau / w
   double wscale;      // dc/dw = -tau / w^2
 };
@@ -167,8 +170,14 @@ __device__ __forceinline__ void bwd_tile(const BwdArgs& a, int tile, int t, BwdL
   constexpr int RY = kBwdRY;
   const int H = a.H, W = a.W, N = H * W;
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
-  const int c0 = tx * G::OX - G::HC, r0 = ty * G::OY - G::HR;
-  const bool inside = c0 >= 0 && r0 >= 0 && c0 + G::LW <= W && r0 + G::LH <= H && (W % 2 == 0);
+  // Edge-aligned placement in both directions (col_place): a tile at the
+  // image edge needs no halo on that side and owns up to it.
+  const ColPlace cp = col_place(tx, a.tiles_x, W, G::LW, G::HC, G::OX);
+  const ColPlace rp = col_place(ty, a.tiles_y, H, G::LH, G::HR, G::OY);
+  const int c0 = cp.c0, r0 = rp.c0;
+  // Tiles that end at the image's last column or row need its gradient mask.
+  const bool inside = c0 >= 0 && r0 >= 0 && c0 + G::LW <= W && r0 + G::LH <= H &&
+                      (W % 2 == 0) && !cp.edge_right && !rp.edge_right;
   if (inside != INSIDE) return;
   LanePos lp;
   lp.w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -176,32 +185,41 @@ __device__ __forceinline__ void bwd_tile(const BwdArgs& a, int tile, int t, BwdL
   lp.row0 = r0 + lp.w * RY;
   lp.col = c0 + 2 * lp.lane;
   const int off0 = lp.row0 * W + lp.col;
-  // Owned: interior of the load tile, inside the image.
-  const int tc = 2 * lp.lane;
-  const bool own_cols = tc >= G::HC && tc < G::LW - G::HC;
+  // Owned: the placement's own range, inside the image.
+  const bool own_cols = lp.col >= cp.own_lo && lp.col < cp.own_hi;
   const bool own_c[2] = {own_cols && lp.col < W, own_cols && lp.col + 1 < W};
   unsigned own_r = 0;
 #pragma unroll
   for (int j = 0; j < RY; ++j) {
-    const int tr = lp.w * RY + j;
-    own_r |= unsigned(tr >= G::HR && tr < G::LH - G::HR && lp.row0 + j < H) << j;
+    const int row = lp.row0 + j;
+    own_r |= unsigned(row >= rp.own_lo && row < rp.own_hi && row < H) << j;
   }
   EdgeMasks em;
   if constexpr (!INSIDE) em.init<RY>(lp, H, W);
 
   // TV input v = x_k + (lambda_k r_k) Phi, formed as the forward pass forms it.
+  // The adjoint of the TV output, xbar = vbar_next + ybb Phi when ybb is
+  // given (the frame part of the next projection's reverse), as one FMA.
   float vr[RY][2], xb[RY][2];
   {
     float xv[RY][2], ph[RY][2];
     load_plane<RY, INSIDE>(a.r, vr, lp, off0, H, W);
     load_plane<RY, INSIDE>(a.x + size_t(t) * N, xv, lp, off0, H, W);
     load_plane<RY, INSIDE>(a.phi + size_t(t) * N, ph, lp, off0, H, W);
+    load_plane<RY, INSIDE>(a.xbar + size_t(t) * N, xb, lp, off0, H, W);
+    if (a.ybb) {
+      float yb[RY][2];
+      load_plane<RY, INSIDE>(a.ybb, yb, lp, off0, H, W);
+#pragma unroll
+      for (int j = 0; j < RY; ++j)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) xb[j][k] = fmaf(yb[j][k], ph[j][k], xb[j][k]);
+    }
 #pragma unroll
     for (int j = 0; j < RY; ++j)
 #pragma unroll
       for (int k = 0; k < 2; ++k) vr[j][k] = fmaf(vr[j][k] * a.lam, ph[j][k], xv[j][k]);
   }
-  load_plane<RY, INSIDE>(a.xbar + size_t(t) * N, xb, lp, off0, H, W);
   // This lane's own slots of the LDS planes (row j at [j * 64]).
   const int slot = lp.w * RY * 64 + lp.lane;
   float2* const vimg = L.vimg + slot;
@@ -337,9 +355,9 @@ __global__ void __launch_bounds__(64 * kBwdBY) k_tv_bwd(BwdArgs a) {
 template <int D>
 int launch_tv_bwd(BwdArgs a, int T, hipStream_t s) {
   using G = BwdGeom<D>;
-  a.tiles_x = (a.W + G::OX - 1) / G::OX;
-  const int tiles_y = (a.H + G::OY - 1) / G::OY;
-  k_tv_bwd<D><<<dim3(unsigned(a.tiles_x * tiles_y), unsigned(T)), 64 * kBwdBY, 0, s>>>(a);
+  a.tiles_x = tiles_across(a.W, G::LW, G::HC, G::OX);
+  a.tiles_y = tiles_across(a.H, G::LH, G::HR, G::OY);
+  k_tv_bwd<D><<<dim3(unsigned(a.tiles_x * a.tiles_y), unsigned(T)), 64 * kBwdBY, 0, s>>>(a);
   SCIGAP_LAUNCH_CHECK("fused TV backward launch");
   return SCIGAP_OK;
 }
@@ -348,16 +366,17 @@ int launch_tv_bwd(BwdArgs a, int T, hipStream_t s) {
 
 // vbar = (d out_D / d v)^T xbar for every frame, and wbar += xbar . d out_D / dw,
 // for D fixed Chambolle updates with weight w on v = x + (lam r) Phi (D in
-// [1, 6]; H*W < 2^30 - 8).
+// [1, 6]; H*W < 2^30 - 8).  With ybb non-null the adjoint is xbar + ybb*Phi.
 int tv_backward_tiles(int D, const float* x, const float* phi, const float* r, float lam,
-                      const float* xbar, float* vbar, int64_t H, int64_t W, int64_t T, float w,
-                      double* wbar, hipStream_t s) {
+                      const float* xbar, const float* ybb, float* vbar, int64_t H, int64_t W,
+                      int64_t T, float w, double* wbar, hipStream_t s) {
   BwdArgs a{};
   a.x = x;
   a.phi = phi;
   a.r = r;
   a.lam = lam;
   a.xbar = xbar;
+  a.ybb = ybb;
   a.vbar = vbar;
   a.wbar = wbar;
   a.H = int(H);

@@ -80,6 +80,8 @@ CASES = [
     (2, 100, 200, 3, 2, 4, False),   # depth 3
     (2, 5, 9, 2, 2, 5, True),        # frames smaller than the reverse tile's halo
     (1, 9, 7, 600, 2, 5, False),     # 600 frames: the frame sums of forward and reverse
+    (1, 300, 500, 2, 2, 5, False),   # edge-aligned reverse tiles, middle tiles both ways
+    (1, 201, 388, 3, 2, 5, True),    # odd height: rows unaligned, columns aligned
 ]
 
 
