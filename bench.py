@@ -647,6 +647,9 @@ def main():
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the batched queue (no comparators) and print its JSON "
                          "(for profiling)")
+    ap.add_argument("--only-other", action="store_true",
+                    help="run only the other solves (ADMM at full size, GAP at 256x256x8) and "
+                         "print their JSON (for profiling)")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
 
@@ -668,6 +671,9 @@ def main():
     H, W, T, K, tvw, mask_kind, noise = CONFIGS[args.config]
     log("generating %s %dx%dx%d" % (args.config, H, W, T))
     prob = make_problem(H, W, T, seed=0, mask=mask_kind, noise_rel=noise)
+    if args.only_other:
+        print(json.dumps({"other_solves": other_solves(prob)}), flush=True)
+        return
     head, solve = headline(args, prob)
 
     if not args.no_traffic and args.config == "2048x2048x20":
