@@ -769,8 +769,12 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
   // requested one stopped frame ahead and its old output before its TV, so
   // that their latency hides behind the stencil arithmetic (as in the main
   // pass); nx stays 0 when X_0 = At(y).
+  // The exit iterations come from LDS (or memory) into VGPRs: readfirstlane
+  // tells the compiler that they, the frame indices and so the plane
+  // pointers are wave-uniform (otherwise every buffer load of a frame is
+  // wrapped in a loop over the lanes' distinct descriptors).
   auto next_stopped = [&](int t) {
-    while (t < T && stop[t] >= D) ++t;
+    while (t < T && __builtin_amdgcn_readfirstlane(stop[t]) >= D) ++t;
     return t;
   };
   int t = next_stopped(0);
@@ -782,7 +786,7 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
   // Every lane reads back only the LDS slots it wrote itself.
   float2* const simg = sh.img + lp.w * RY * 64 + lp.lane;
   while (t < T) {
-    const int nup = stop[t];
+    const int nup = __builtin_amdgcn_readfirstlane(stop[t]);
     float ph[RY][2];
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
