@@ -61,24 +61,30 @@ def test_batch_matches_single_and_oracle(gpu, method, kw):
         assert rel(np.moveaxis(xb, 0, -1), want) <= 1e-4
 
 
-def test_batch_with_many_frames(gpu):
+@pytest.mark.parametrize("method", ["gap", "admm"])
+@pytest.mark.parametrize("n", [5, 6, 7])
+def test_batch_with_many_frames(gpu, method, n):
     """A queue holding a 1030-frame solve beside an ordinary one: the batched
     init and fused kernels accumulate the frame sums in fp64 like the single
-    solve (eps = 1 makes every frame stop at the first exit test in FP32 and
-    FP64 alike, so the comparison cannot hinge on a borderline decision)."""
+    solve -- in registers for tv_iter_max 5, as fp32 partials over blocks of 8
+    frames added to fp64 LDS slots for 6 and 7 (eps = 1 makes every frame stop
+    at the first exit test in FP32 and FP64 alike, so the comparison cannot
+    hinge on a borderline decision)."""
     from pnp_sci.batch import BatchPlan
     from pnp_sci.synthetic import make_problem
     probs = [make_problem(9, 7, 1030, seed=6, p=0.5), make_problem(48, 80, 5, seed=1, p=0.5)]
     dev = [_device(p) for p in probs]
     iters = 4
-    plan = BatchPlan(dev, iters=iters, tv_weight=0.1, tv_eps=1.0, use_graph=False)
+    plan = BatchPlan(dev, method=method, iters=iters, tv_weight=0.1, tv_iter_max=n, tv_eps=1.0,
+                     use_graph=False)
     xs = [x.cpu().numpy() for x in plan.run()]
     plan.close()
     for pr, xb in zip(probs, xs):
         m = pr.mask.astype(np.float64)
-        want = O.gap_denoise(pr.y.astype(np.float64), pr.phi_sum.astype(np.float64),
-                             lambda v: O.A_(v, m), lambda v: O.At_(v, m), iter_max=iters,
-                             tv_weight=0.1, show_iqa=False, _tv_eps=1.0)[0]
+        run = O.gap_denoise if method == "gap" else O.admm_denoise
+        want = run(pr.y.astype(np.float64), pr.phi_sum.astype(np.float64),
+                   lambda v: O.A_(v, m), lambda v: O.At_(v, m), iter_max=iters,
+                   tv_weight=0.1, tv_iter_max=n, show_iqa=False, _tv_eps=1.0)[0]
         assert rel(np.moveaxis(xb, 0, -1), want) <= 1e-4
 
 
