@@ -24,11 +24,12 @@
 // iteration before it (and every iteration when per-iteration errors are
 // requested, since x_k = v_k + b_k).
 //
-// Early exit.  Each launch accumulates the per-frame energies E_0..E_{D-1};
-// a one-workgroup decision kernel derives each frame's exit iteration, and a
-// repair kernel (which returns at once when no frame stopped early)
-// recomputes the frames that stopped at s < D from the unchanged inputs
-// (x, r and y1 are double-buffered) and corrects yb and the epilogue.
+// Early exit.  Each launch stores per-tile partial energies E_0..E_{D-1} of
+// every frame; a decision kernel (one workgroup per frame) sums them in a
+// fixed order and derives each frame's exit iteration, and a repair kernel
+// (which returns at once when no frame stopped early) recomputes the frames
+// that stopped at s < D from the unchanged inputs (x, r and y1 are
+// double-buffered) and corrects yb and the epilogue.
 
 #include <algorithm>
 #include <atomic>
