@@ -252,7 +252,7 @@ int launch_fused(const FusedArgs& a00, hipStream_t s, int tile_rows, int engine)
   const unsigned grid = set_tiles<D, FBY, MR>(a);
   {
     ProfScope ps("fused_decide", s);
-    k_decide<D><<<unsigned(a.T), 256, 0, s>>>(a);
+    k_decide<D><<<unsigned(a.T), kDecideThreads, 0, s>>>(a);
     SCIGAP_LAUNCH_CHECK("early-exit decision launch");
   }
   {
