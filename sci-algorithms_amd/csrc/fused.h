@@ -508,9 +508,8 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
     float out[RY][2];
     float acc[NH * 8];
     if constexpr (X1) {
-#pragma unroll
-      for (int u = 0; u < D; ++u) prefetch(u);
-      tv_run_x1<D, RY, BY, NH, !INSIDE>(img, out, D, lds, lp, em, a.weight, a.tw, acc, true);
+      tv_run_x1<D, RY, BY, NH, !INSIDE>(img, out, D, lds, lp, em, a.weight, a.tw, acc, true,
+                                        prefetch);
     } else
       tv_run_from_zero<D, RY, BY, NH, !INSIDE>(img, out, D, lds, lp, em, a.weight, a.tw, acc,
                                                true, prefetch);

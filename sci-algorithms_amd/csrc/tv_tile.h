@@ -330,12 +330,13 @@ __device__ __forceinline__ void tv_div_regs(const float2* img, float (&out)[RY][
 
 // tv_run_from_zero with one barrier per update (plus one at entry, after
 // which the img rows of the wave below are read).  Same arguments and
-// results, bit for bit.
-template <int D, int RY, int BY, int NH, bool EDGE>
+// results, bit for bit; hook(u) runs after the dual step of update u.
+template <int D, int RY, int BY, int NH, bool EDGE, class Hook = NoHook>
 __device__ __forceinline__ void tv_run_x1(const float2* img, float (&out)[RY][2], int nup,
                                           TileLdsX1<BY>& lds, const LanePos& lp,
                                           const EdgeMasks& em, float weight, float tw,
-                                          float (&acc)[NH * 8], bool accumulate) {
+                                          float (&acc)[NH * 8], bool accumulate,
+                                          const Hook& hook = Hook()) {
   float p0[RY][2], p1[RY][2];
 #pragma unroll
   for (int j = 0; j < RY; ++j) {
@@ -365,6 +366,7 @@ __device__ __forceinline__ void tv_run_x1(const float2* img, float (&out)[RY][2]
       else
         tv_grad_regs<RY, NH, EDGE>(out, below, p0, p1, em, weight, tw,
                                    accumulate ? &acc[u] : nullptr);
+      hook(u);
       const int par = u & 1;
       lds.last0[par][me] = make_float2(p0[RY - 1][0], p0[RY - 1][1]);
       lds.first0[par][me] = make_float2(p0[0][0], p0[0][1]);
