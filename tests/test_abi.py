@@ -97,7 +97,7 @@ def test_unrolled_sizes_and_validation(dll):
     ck = dll.scigap_unrolled_checkpoint_bytes(ctypes.byref(u))
     assert ck >= 2 * (3 * V * 4 + 3 * N * 4 + N * 4)      # x_k, r_k, y per sample
     ws = dll.scigap_unrolled_workspace_bytes(ctypes.byref(u))
-    assert ws >= 2 * 3 * V * 4                            # two lanes: vbar, xbar x2 each
+    assert ws >= 2 * 2 * V * 4                            # two lanes: vbar x2 each
     for field, bad in (("tv_iter_max", 8), ("tv_iter_max", 1), ("iters", 0), ("batch", 0)):
         v = _lib.Unrolled.from_buffer_copy(u)
         setattr(v, field, bad)
