@@ -1085,6 +1085,15 @@ __global__ void __launch_bounds__(256) k_gap_epi(FusedArgs a) {
   }
 }
 
+// *out += number of frames whose TV returned before its last iteration
+// (stop[t] < last); integer sums, so the count does not depend on order.
+__global__ void __launch_bounds__(256) k_count_early(const int* __restrict__ stop, int T,
+                                                     int last, int* out) {
+  int c = 0;
+  for (int t = int(threadIdx.x); t < T; t += blockDim.x) c += stop[t] < last ? 1 : 0;
+  if (c) atomicAdd(out, c);
+}
+
 __global__ void k_fill_i32(int* p, int n, int v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;

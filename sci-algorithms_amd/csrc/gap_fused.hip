@@ -515,8 +515,13 @@ int scigap_solve(const scigap_params* p, const float* y, const float* phi, const
       k_gap_proj<<<grid_pixels(N), 256, 0, s>>>(pa);
       SCIGAP_LAUNCH_CHECK("projection launch");
       if (int rc = tv_chunked(ws.vbuf, a.x_out, p->H, p->W, T, a.weight, p->tv_eps,
-                              p->tv_iter_max, deep_depth, nullptr, ws.tvws, s))
+                              p->tv_iter_max, deep_depth, stats ? ws.st.stop : nullptr,
+                              ws.tvws, s))
         return rc;
+      if (stats) {  // frames whose TV stopped early in this iteration
+        k_count_early<<<1, 256, 0, s>>>(ws.st.stop, int(T), p->tv_iter_max - 1, stats + k);
+        SCIGAP_LAUNCH_CHECK("early-stop count launch");
+      }
       FusedArgs ea = a;
       ea.x_in = ws.vbuf;
       k_gap_epi<<<grid_pixels(N), 256, 0, s>>>(ea);

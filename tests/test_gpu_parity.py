@@ -654,3 +654,35 @@ def test_random_shapes_engines_against_fp64_oracle(gpu):
         e = rel(got, want)
         print("case", case, (H, W, T, method, n, rows, engine), "rel %.2e" % e)
         assert e <= 1e-4, (case, H, W, T, method, n, rows, engine, e)
+
+
+@pytest.mark.parametrize("method,n", [("gap", 12), ("admm", 30)])
+def test_deep_tv_solve_reports_early_stops(gpu, method, n):
+    """Solves whose TV runs through the chunked deep path (tv_iter_max > 7)
+    report the per-iteration early-stop counts like the fused path: equal to
+    the FP64 C oracle's (ADMM: all but the last iteration), and the result
+    within 1e-4 of it."""
+    import torch
+    from pnp_sci import _lib
+    from pnp_sci.device import to_device
+    oracle_c = pytest.importorskip("oracle_c", reason="make -C oracle builds the C oracle")
+    H, W, T, K = 40, 70, 9, 5
+    pr = problem(H, W, T, seed=21)
+    want, ex = oracle_c.gap_tv(pr.y, pr.mask, pr.phi_sum, iters=K, method=method, tv_weight=0.1,
+                               eps=2e-2, tv_iter_max=n, exits=True)
+    L = gpu
+    p = _lib.Params()
+    L.scigap_default_params(p, H, W, T)
+    p.method = _lib.METHOD_ADMM if method == "admm" else _lib.METHOD_GAP
+    p.iters, p.tv_weight, p.tv_iter_max, p.tv_eps = K, 0.1, n, 2e-2
+    y, phi, ps = to_device(pr.y), to_device(pr.mask_thw), to_device(pr.phi_sum)
+    nb = L.scigap_solve_workspace_bytes(p)
+    ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    x = torch.empty((T, H, W), device="cuda")
+    st = torch.zeros(K, dtype=torch.int32, device="cuda")
+    _lib.check(L.scigap_solve(p, y.data_ptr(), phi.data_ptr(), ps.data_ptr(), None, x.data_ptr(),
+                              None, None, st.data_ptr(), ws.data_ptr(), nb, _lib.stream_ptr()))
+    got = st.cpu().tolist()
+    m = K - 1 if method == "admm" else K
+    assert 0 < sum(ex[:m]) and got[:m] == ex[:m].tolist(), (got, ex.tolist())
+    assert rel(x.cpu().numpy().transpose(1, 2, 0), want) <= 1e-4
