@@ -260,8 +260,6 @@ int check_batch(const scigap_params* p, const scigap_batch_item* items, int n) {
   if (D < 1 || D > kMaxBatchDepth)
     return fail(SCIGAP_EUNSUPPORTED, "batch: tv_iter_max must be in [2, %d]", kMaxBatchDepth + 1);
   if (p->iters < 1) return fail(SCIGAP_EINVAL, "batch: iters must be >= 1");
-  if (p->method == SCIGAP_METHOD_ADMM && p->iters < 2)
-    return fail(SCIGAP_EUNSUPPORTED, "batch: ADMM needs iters >= 2");
   return SCIGAP_OK;
 }
 
@@ -417,7 +415,9 @@ int scigap_batch_plan_create(scigap_batch_plan** out, const scigap_params* p,
     const scigap_batch_item& it = items[i];
     int tiles_x = 0;
     const int ntiles = batch_tiles(it.H, it.W, g, &tiles_x);
-    for (int k = 0; k < P->KF; ++k) {
+    // (ADMM with one iteration runs no fused iteration; its k = 0 arguments
+    // still drive the initial residual.)
+    for (int k = 0; k < std::max(P->KF, 1); ++k) {
       FusedArgs a = iteration_args(*p, k, it.H, it.W, it.T, it.y, it.phi, it.phi_sum, it.x0,
                                    it.x_out, nullptr, nullptr, nullptr, L.st[i]);
       a.tiles_x = tiles_x;
@@ -429,10 +429,11 @@ int scigap_batch_plan_create(scigap_batch_plan** out, const scigap_params* p,
     if (P->mode == ADMM) {
       const int K = p->iters;
       FusedArgs f{};
-      f.from_y = 0;
-      f.x_in = state_buffer(*p, L.st[i], it.x_out, K - 1);
+      // One iteration: x = X_0 + lambda*r_0*Phi with X_0 = x0 or At(y), b = 0.
+      f.from_y = (K == 1 && !it.x0) ? 1 : 0;
+      f.x_in = K == 1 ? it.x0 : state_buffer(*p, L.st[i], it.x_out, K - 1);
       f.x_out = it.x_out;
-      f.b = L.st[i].bbuf;
+      f.b = K == 1 ? nullptr : L.st[i].bbuf;
       f.r_in = L.st[i].r[(K - 1) & 1];
       f.phi = it.phi;
       f.y = it.y;
@@ -488,7 +489,7 @@ int scigap_batch_plan_create(scigap_batch_plan** out, const scigap_params* p,
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
   };
   hipError_t e = hipSuccess;
-  if (P->KF > 0) e = up(L.descs, descs.data(), descs.size() * sizeof(FusedArgs));
+  e = up(L.descs, descs.data(), descs.size() * sizeof(FusedArgs));
   if (e == hipSuccess && P->mode == ADMM) e = up(L.fin, fin.data(), fin.size() * sizeof(FusedArgs));
   if (e == hipSuccess) e = up(L.queue, queue.data(), queue.size() * sizeof(int2));
   if (e == hipSuccess) e = up(L.qoff, qoff.data(), qoff.size() * sizeof(int));

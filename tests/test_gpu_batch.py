@@ -163,3 +163,28 @@ def test_batch_autotune_picks_and_remembers(gpu):
                                               accelerate=True, gamma=0.01, clip=False,
                                               tv_weight=0.1, tv_iter_max=5,
                                               tv_eps=2e-4)) is None
+
+
+@pytest.mark.parametrize("method", ["gap", "admm"])
+def test_batch_one_iteration(gpu, method):
+    """One outer iteration in a batch (ADMM then runs no fused iteration: its
+    result is the projection X_0 + lambda*r_0*Phi), with and without a warm
+    start, against the FP64 oracle."""
+    from pnp_sci.batch import BatchPlan
+    import torch
+    probs = _problems(seed=5)[:3]
+    dev = [_device(p) for p in probs]
+    rng = np.random.default_rng(5)
+    x0s = [rng.random(p.mask_thw.shape).astype(np.float32) for p in probs]
+    dev[1] = (dev[1][0], dev[1][1], dev[1][2], torch.from_numpy(x0s[1]).cuda())
+    plan = BatchPlan(dev, method=method, iters=1, tv_weight=0.1, use_graph=False)
+    xs = [x.cpu().numpy() for x in plan.run()]
+    plan.close()
+    for i, (pr, xb) in enumerate(zip(probs, xs)):
+        m = pr.mask.astype(np.float64)
+        run = O.gap_denoise if method == "gap" else O.admm_denoise
+        x0 = np.moveaxis(x0s[1], 0, -1).astype(np.float64) if i == 1 else None
+        want = run(pr.y.astype(np.float64), pr.phi_sum.astype(np.float64),
+                   lambda v: O.A_(v, m), lambda v: O.At_(v, m), iter_max=1, tv_weight=0.1,
+                   x0=x0, show_iqa=False)[0]
+        assert rel(np.moveaxis(xb, 0, -1), want) <= 1e-5, i
