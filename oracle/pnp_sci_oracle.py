@@ -169,7 +169,9 @@ def tv_chambolle_nd(image, weight=0.1, eps=2.e-4, n_iter_max=200, _stats=None):
       of iteration n_iter_max-1, i.e. after n_iter_max-1 dual updates.
 
     ``_stats`` (a dict, test hook) receives ``returned``: the iteration whose
-    ``out`` is returned.
+    ``out`` is returned, and ``margin``: the smallest distance of an exit test
+    from its threshold, min |(|E_prev - E| / (eps*E_0)) - 1| over the tests
+    taken (inf when none was), which tells a decision that rounding could flip.
     """
     nd = image.ndim
     dt = image.dtype
@@ -182,6 +184,7 @@ def tv_chambolle_nd(image, weight=0.1, eps=2.e-4, n_iter_max=200, _stats=None):
     d = np.zeros_like(image)
     out = image
     e_first = e_prev = 0.0
+    margin = float("inf")
     it = 0
     while it < n_iter_max:
         if it > 0:
@@ -207,13 +210,16 @@ def tv_chambolle_nd(image, weight=0.1, eps=2.e-4, n_iter_max=200, _stats=None):
         energy /= float(image.size)
         if it == 0:
             e_first = e_prev = energy
-        elif abs(e_prev - energy) < eps * e_first:
-            break
         else:
+            if _stats is not None and eps * e_first > 0:
+                margin = min(margin, abs(abs(e_prev - energy) / (eps * e_first) - 1.0))
+            if abs(e_prev - energy) < eps * e_first:
+                break
             e_prev = energy
         it += 1
     if _stats is not None:
         _stats["returned"] = min(it, n_iter_max - 1)
+        _stats["margin"] = margin
     return out
 
 

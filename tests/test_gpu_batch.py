@@ -188,3 +188,33 @@ def test_batch_one_iteration(gpu, method):
                    lambda v: O.A_(v, m), lambda v: O.At_(v, m), iter_max=1, tv_weight=0.1,
                    x0=x0, show_iqa=False)[0]
         assert rel(np.moveaxis(xb, 0, -1), want) <= 1e-5, i
+
+
+def test_random_batches_against_fp64_c_oracle(gpu):
+    """Seeded random batches (1-8 solves of random sizes, GAP or ADMM, one to
+    six iterations, TV depths 2-7, exit tolerances, tile heights, queue order,
+    graph or plain launches): every solve within 1e-4 of the FP64 C oracle."""
+    from pnp_sci.batch import BatchPlan
+    from pnp_sci.synthetic import make_problem
+    oracle_c = pytest.importorskip("oracle_c", reason="make -C oracle builds the C oracle")
+    rng = np.random.default_rng(3)
+    for case in range(20):
+        method = ["gap", "admm"][int(rng.integers(0, 2))]
+        n, K = int(rng.integers(2, 8)), int(rng.integers(1, 7))
+        eps, w = float(rng.choice([2e-4, 2e-2, 1.0])), float(rng.choice([0.05, 0.1, 0.3]))
+        rows = int(rng.choice([0, 8, 12, 16])) if n == 5 else 0
+        graph, order = bool(rng.integers(0, 2)), int(rng.integers(0, 2))
+        probs = [make_problem(int(rng.integers(1, 300)), int(rng.integers(1, 300)),
+                              int(rng.integers(1, 25)), seed=100 * case + i,
+                              p=float(rng.uniform(0.3, 0.7)))
+                 for i in range(int(rng.integers(1, 9)))]
+        plan = BatchPlan([_device(pr) for pr in probs], method=method, iters=K, tv_weight=w,
+                         tv_iter_max=n, tv_eps=eps, use_graph=graph, tile_rows=rows, grid=0,
+                         order=order)
+        xs = [x.cpu().numpy() for x in plan.run()]
+        plan.close()
+        for i, (pr, xb) in enumerate(zip(probs, xs)):
+            want = oracle_c.gap_tv(pr.y, pr.mask, pr.phi_sum, iters=K, method=method,
+                                   tv_weight=w, eps=eps, tv_iter_max=n)
+            e = rel(np.moveaxis(xb, 0, -1), want)
+            assert e <= 1e-4, (case, i, pr.mask.shape, method, n, K, eps, w, rows, graph, e)

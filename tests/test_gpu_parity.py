@@ -686,3 +686,85 @@ def test_deep_tv_solve_reports_early_stops(gpu, method, n):
     m = K - 1 if method == "admm" else K
     assert 0 < sum(ex[:m]) and got[:m] == ex[:m].tolist(), (got, ex.tolist())
     assert rel(x.cpu().numpy().transpose(1, 2, 0), want) <= 1e-4
+
+
+def _closest_exit_test(pr, method, K, n, eps, w, monkeypatch):
+    """Smallest distance of any TV exit test of the FP64 solve from its
+    threshold (relative to eps*E_0), from the NumPy oracle's stats hook."""
+    margins = []
+    tv64 = O.tv_chambolle_nd
+
+    def tv(image, weight=0.1, eps=2e-4, n_iter_max=200, _stats=None):
+        st = {}
+        out = tv64(image, weight, eps, n_iter_max, _stats=st)
+        margins.append(st["margin"])
+        return out
+    with monkeypatch.context() as mp:
+        mp.setattr(O, "tv_chambolle_nd", tv)
+        A64, At64 = ops64(pr.mask)
+        args = (pr.y.astype(np.float64), pr.phi_sum.astype(np.float64), A64, At64)
+        kw = dict(iter_max=K, tv_weight=w, tv_iter_max=n, show_iqa=False, _tv_eps=eps)
+        if method == "admm":
+            O.admm_denoise(*args, **kw)
+        else:
+            O.gap_denoise(*args, accelerate=(method == "gap"), **kw)
+    return min(margins) if margins else float("inf")
+
+
+def test_wide_random_sweep_against_fp64_c_oracle(gpu, monkeypatch):
+    """160 seeded solves over random shapes (widths around the column tiles),
+    frame counts, GAP (accelerated and plain) and ADMM, TV depths from 2 to 30
+    (fused and chunked), exit tolerances, weights, tile heights and engines:
+    the per-iteration early-stop counts equal the FP64 C oracle's and every
+    solve is within 1e-4 of it -- except where an exit test of the FP64 solve
+    lies within 1e-3 of its threshold, so that FP32 energies may decide the
+    other way (then within 5e-3, at most 4 such solves)."""
+    import torch
+    from pnp_sci import _lib
+    from pnp_sci.device import to_device
+    oracle_c = pytest.importorskip("oracle_c", reason="make -C oracle builds the C oracle")
+    L = gpu
+    rng = np.random.default_rng(7)
+    widths = [1, 2, 5, 7, 64, 100, 127, 128, 129, 130, 240, 241, 250, 255, 256, 257, 383, 500]
+    worst, borderline = 0.0, []
+    for case in range(160):
+        H, W, T = int(rng.integers(1, 260)), int(rng.choice(widths)), int(rng.integers(1, 40))
+        method = ["gap", "gap_plain", "admm"][int(rng.integers(0, 3))]
+        n = int(rng.choice([2, 3, 4, 5, 6, 7, 9, 12, 30]))
+        K = int(rng.integers(1, 9))
+        eps = float(rng.choice([2e-4, 2e-3, 2e-2, 1.0]))
+        rows = int(rng.choice([0, 8, 12, 16])) if n == 5 else 0
+        engine = int(rng.integers(0, 7)) if n == 5 else 0
+        w = float(rng.choice([0.02, 0.1, 0.3]))
+        pr = problem(H, W, T, seed=case, p=float(rng.uniform(0.3, 0.7)))
+        want, ex = oracle_c.gap_tv(pr.y, pr.mask, pr.phi_sum, iters=K, method=method,
+                                   tv_weight=w, eps=eps, tv_iter_max=n, exits=True)
+        p = _lib.Params()
+        L.scigap_default_params(p, H, W, T)
+        p.method = _lib.METHOD_ADMM if method == "admm" else _lib.METHOD_GAP
+        p.accelerate = 0 if method == "gap_plain" else 1
+        p.iters, p.tv_weight, p.tv_iter_max, p.tv_eps = K, w, n, eps
+        p.tile_rows, p.engine = rows, engine
+        y, phi, ps = to_device(pr.y), to_device(pr.mask_thw), to_device(pr.phi_sum)
+        nb = L.scigap_solve_workspace_bytes(p)
+        ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        x = torch.empty((T, H, W), device="cuda")
+        st = torch.zeros(K, dtype=torch.int32, device="cuda")
+        _lib.check(L.scigap_solve(p, y.data_ptr(), phi.data_ptr(), ps.data_ptr(), None,
+                                  x.data_ptr(), None, None, st.data_ptr(), ws.data_ptr(), nb,
+                                  _lib.stream_ptr()))
+        m = K - 1 if method == "admm" else K  # ADMM's last iteration runs no TV
+        label = (case, H, W, T, method, n, K, eps, w, rows, engine)
+        same = st.cpu().tolist()[:m] == ex[:m].tolist()
+        e = rel(x.cpu().numpy().transpose(1, 2, 0), want)
+        if same and e <= 1e-4:
+            worst = max(worst, e)
+            continue
+        margin = _closest_exit_test(pr, method, K, n, eps, w, monkeypatch)
+        print("case", label, "rel %.2e, stop counts equal: %s, closest exit test %.2e" % (
+            e, same, margin))
+        assert margin < 1e-3 and e <= 5e-3, (label, e, same, margin)
+        borderline.append(label)
+    print("160 random solves: worst relative error vs FP64 %.2e with every exit test clear of "
+          "its threshold; borderline: %s" % (worst, borderline))
+    assert len(borderline) <= 4
