@@ -409,3 +409,28 @@ def test_borderline_case_fp32_drift():
     x32, x64 = res[np.float32].astype(np.float64), res[np.float64]
     drift = np.linalg.norm(x32 - x64) / np.linalg.norm(x64)
     assert 1.0e-3 < drift < 1.5e-3, drift
+
+
+def test_tv_exit_margin_reaches_zero_where_the_decision_flips():
+    """The stats hook's ``margin`` (distance of the closest exit test from its
+    threshold) goes to zero exactly where a change of eps changes the returned
+    iteration: bisecting eps across such a change leaves both ends with a
+    margin that shrinks with the bracket."""
+    rng = np.random.default_rng(4)
+    img = rng.random((12, 9))
+
+    def returned(eps):
+        st = {}
+        O.tv_chambolle_nd(img, 0.1, eps, 30, _stats=st)
+        return st["returned"], st["margin"]
+    lo, hi = 1e-5, 1e-1
+    r_lo, r_hi = returned(lo)[0], returned(hi)[0]
+    assert r_lo > r_hi  # a larger eps stops earlier
+    for _ in range(50):
+        mid = 0.5 * (lo + hi)
+        if returned(mid)[0] == r_lo:
+            lo = mid
+        else:
+            hi = mid
+    assert returned(lo)[1] < 1e-9 and returned(hi)[1] < 1e-9
+    assert returned(1e-5)[1] > 1e-3
