@@ -347,6 +347,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
   // Interior tiles may prefetch the next frame straight into LDS.
   constexpr bool kDma = DMA && INSIDE && (RY % 2 == 0);
   auto& lds = sh.eng;
+  if constexpr (!X1) lds.zero_edges();
   float* const red = sh.red;
   double* const red64 = sh.red64;
   const int H = a.H, W = a.W;
@@ -742,6 +743,7 @@ __device__ __forceinline__ void fused_fix_tile(const FusedArgs& a, int tile, Fix
   constexpr int NH = MR;
   constexpr bool kTvOnly = MODE == TV_ONLY;
   TileLds<BY>& lds = sh.eng;
+  lds.zero_edges();
   double* const red64 = sh.red64;
   const int H = a.H, W = a.W, T = a.T;
   const int N = H * W;

@@ -101,6 +101,7 @@ __device__ __forceinline__ void chunk_tile(const ChunkArgs& a, int tile, ChunkLd
   EdgeMasks em;
   if constexpr (!INSIDE) em.init<RY>(lp, H, W);
   float2* const img = sh.img + lp.w * RY * 64 + lp.lane;
+  sh.eng.zero_edges();
   if constexpr (!REPAIR)
     for (int i = threadIdx.x; i < kEnergyFrames * BY * EnergySlots<D>::kPerWave; i += blockDim.x)
       sh.red[i] = 0.f;

@@ -50,11 +50,24 @@ struct TileGeom {
   static_assert(D >= 1 && D <= 8, "fused depth out of range");
 };
 
-// LDS used by the engine: one row of `out` and one row of `p0` per wave.
+// LDS used by the engine: one row of `out` per wave (row w) followed by a
+// zero row, and a zero row followed by one row of `p0` per wave (row w+1), so
+// that every wave reads its neighbours' rows without a branch: the first
+// wave's row above and the last wave's row below are the zero rows.  Those
+// two rows hold a halo strip's outermost rows only, whose values the tile
+// discards (the first wave's above is also the reference's p = 0 beyond the
+// top edge).  zero_edges() writes them; every kernel calls it before the
+// engine's first barrier.
 template <int BY>
 struct TileLds {
-  float xo[BY * 128];
-  float xp[BY * 128];
+  float xo[(BY + 1) * 128];
+  float xp[(BY + 1) * 128];
+  __device__ void zero_edges() {
+    if (threadIdx.x < 64) {
+      reinterpret_cast<float2*>(xo)[BY * 64 + threadIdx.x] = make_float2(0.f, 0.f);
+      reinterpret_cast<float2*>(xp)[threadIdx.x] = make_float2(0.f, 0.f);
+    }
+  }
 };
 
 // Per-lane view of where its pixels are (for masks and ownership).
@@ -106,14 +119,14 @@ __device__ __forceinline__ void tv_div(const float2* img, float (&out)[RY][2],
                                        TileLds<BY>& lds, const LanePos& lp, float* dacc) {
   // Publish this wave's last row of p0 for the wave below.
   float2* xp = reinterpret_cast<float2*>(lds.xp);
-  xp[lp.w * 64 + lp.lane] = make_float2(p0[RY - 1][0], p0[RY - 1][1]);
+  xp[(lp.w + 1) * 64 + lp.lane] = make_float2(p0[RY - 1][0], p0[RY - 1][1]);
   // The img slots are this lane's own: read them before the barrier, so that
   // their latency overlaps the wait instead of following it.
   float2 im[RY];
 #pragma unroll
   for (int j = 0; j < RY; ++j) im[j] = img[j * 64];
   __syncthreads();
-  float2 above = lp.w > 0 ? xp[(lp.w - 1) * 64 + lp.lane] : make_float2(0.f, 0.f);
+  const float2 above = xp[lp.w * 64 + lp.lane];
   // Rows 1.. first: only row 0 needs the row of the wave above, whose read
   // is still in flight.  The squares are summed in row order afterwards.
   float d0[RY], d1[RY];
@@ -153,8 +166,7 @@ __device__ __forceinline__ void tv_grad(const float (&out)[RY][2], float (&p0)[R
   float2* xo = reinterpret_cast<float2*>(lds.xo);
   xo[lp.w * 64 + lp.lane] = make_float2(out[0][0], out[0][1]);
   __syncthreads();
-  float2 below = lp.w + 1 < BY ? xo[(lp.w + 1) * 64 + lp.lane]
-                               : make_float2(out[RY - 1][0], out[RY - 1][1]);
+  const float2 below = xo[(lp.w + 1) * 64 + lp.lane];
   float s[NH] = {};
 #pragma unroll
   for (int j = 0; j < RY; ++j) {
@@ -175,7 +187,8 @@ __device__ __forceinline__ void tv_grad(const float (&out)[RY][2], float (&p0)[R
     float n0 = fast_sqrt(fmaf(g00, g00, g10 * g10));
     float n1 = fast_sqrt(fmaf(g01, g01, g11 * g11));
     const int h = (NH == 2 && j >= RY / 2) ? 1 : 0;
-    s[h] += n0 + n1;
+    // The first row of a half assigns (0 + x would be an instruction).
+    s[h] = j == h * (RY / 2) ? n0 + n1 : s[h] + (n0 + n1);
     float i0 = fast_rcp(fmaf(tw, n0, 1.f));
     float i1 = fast_rcp(fmaf(tw, n1, 1.f));
     p0[j][0] = (ZERO ? g00 : p0[j][0] + g00) * i0;
@@ -283,7 +296,8 @@ __device__ __forceinline__ void tv_grad_regs(const float (&out)[RY][2], float2 b
     float n0 = fast_sqrt(fmaf(g00, g00, g10 * g10));
     float n1 = fast_sqrt(fmaf(g01, g01, g11 * g11));
     const int h = (NH == 2 && j >= RY / 2) ? 1 : 0;
-    s[h] += n0 + n1;
+    // The first row of a half assigns (0 + x would be an instruction).
+    s[h] = j == h * (RY / 2) ? n0 + n1 : s[h] + (n0 + n1);
     float i0 = fast_rcp(fmaf(tw, n0, 1.f));
     float i1 = fast_rcp(fmaf(tw, n1, 1.f));
     p0[j][0] = (ZERO ? g00 : p0[j][0] + g00) * i0;
