@@ -317,15 +317,18 @@ __device__ __forceinline__ void dma_rows(const float* plane, int N, float* rows,
 // Callers run it under `if (tp.own_c[0])` (per lane); owned rows are
 // wave-uniform.  The row offset is formed here, from a base that is made
 // opaque once per frame, so that the compiler does not keep RY per-row
-// offsets in registers across the frame loop.
-template <int D, int BY, int MR, bool INSIDE, int RY>
+// offsets in registers across the frame loop.  ALL_ROWS: the caller knows
+// that the wave owns every one of its rows (no per-row tests).
+template <int D, int BY, int MR, bool INSIDE, int RY, bool ALL_ROWS = false>
 __device__ __forceinline__ void store_owned(float* __restrict__ p, int N, int W,
                                             const TilePlace<D, BY, MR>& tp, unsigned base,
                                             const float (&v)[RY][2]) {
   const __amdgpu_buffer_rsrc_t rs = plane_rsrc(p, N);
 #pragma unroll
   for (int j = 0; j < RY; ++j) {
-    if (!((tp.own_r >> j) & 1u)) continue;
+    if constexpr (!ALL_ROWS) {
+      if (!((tp.own_r >> j) & 1u)) continue;
+    }
     const unsigned o = base + unsigned(j * W) * 4u;
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v[j][0]), rs, o, 0, 0);
     if (INSIDE || tp.own_c[1])
@@ -485,10 +488,11 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
     // dual updates (their latency then hides behind the rest of this frame's
     // stencil), so that the requests of the eight waves do not all meet the
     // first barrier of the frame.
-    const bool more = t + 1 < T;
+    // On the last frame the requests read the current frame again (in bounds,
+    // and still in the L2 cache), so that no branch guards them.
+    const int nstep = t + 1 < T ? N : 0;
     auto prefetch = [&](int u) {
       if constexpr (!kDma) {
-        if (!more) return;
         // rows [lo, hi) of a plane go out after update u: Phi first, then x
         constexpr int S = kPrefetchSpread < D ? kPrefetchSpread : D;
         constexpr int total = 2 * RY;
@@ -498,9 +502,10 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
         for (int r = 0; r < total; ++r) {
           if (r < lo || r >= hi) continue;
           if (r < RY) {
-            if constexpr (MODE != TV_ONLY) load_row<RY, INSIDE>(phit + N, nph, lp, off0, H, W, r);
+            if constexpr (MODE != TV_ONLY)
+              load_row<RY, INSIDE>(phit + nstep, nph, lp, off0, H, W, r);
           } else if (!from_y) {
-            load_row<RY, INSIDE>(xt + N, nx, lp, off0, H, W, r - RY);
+            load_row<RY, INSIDE>(xt + nstep, nx, lp, off0, H, W, r - RY);
           }
         }
       }
@@ -557,9 +562,12 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
       if (tp.own_c[0]) {
         unsigned base = unsigned(off0) * 4u;
         compute_here(base);
-        store_owned<D, BY, MR, INSIDE>(xo, N, W, tp, base, out);
+        // In an interior tile of one-strip waves (MR = 1) a wave that owns
+        // any row owns all of them: strips 1 .. BY-2 are the tile interior.
+        constexpr bool kAllRows = INSIDE && MR == 1;
+        store_owned<D, BY, MR, INSIDE, RY, kAllRows>(xo, N, W, tp, base, out);
         if constexpr (MODE == ADMM) {
-          if (a.write_b) store_owned<D, BY, MR, INSIDE>(bt, N, W, tp, base, bv);
+          if (a.write_b) store_owned<D, BY, MR, INSIDE, RY, kAllRows>(bt, N, W, tp, base, bv);
         }
       }
     }
