@@ -70,28 +70,44 @@ struct BwdLds {
   float2 vb[kBwdBY * kBwdRY * 64];    // vbar
   float2 pb0[kBwdBY * kBwdRY * 64];   // pbar (vertical component)
   float2 pb1[kBwdBY * kBwdRY * 64];   // pbar (horizontal component)
-  float2 x[2][kBwdBY * 64];           // row exchanges
+  float2 x[2][(kBwdBY + 2) * 64];     // row exchanges: zero row, wave rows, zero row
   float red[kBwdBY];
 };
 
 // Row exchange between neighbouring waves: publish this wave's row `mine`
-// and return the one wave w+dir published (`edge` when there is none).
-// Consecutive exchanges alternate buffers, and every exchange has a barrier
-// between its write and its reads, so a buffer is rewritten only after all
-// reads of its previous contents.
+// and return the one wave w+dir published.  Wave w writes row w+1 of a
+// buffer whose first and last rows stay zero, so the first wave's row above
+// and the last wave's row below are zeros, read without a branch: above the
+// tile's first row that is the reference's p = 0 beyond the top edge (a tile
+// starts either there or 2D rows of halo inside the image), and the last
+// wave's last row is halo or masked (a tile that ends at the image's last row
+// is a border tile, whose gradient mask zeroes that difference), so the row
+// below never reaches an owned pixel.  Consecutive exchanges alternate
+// buffers, and every exchange has a barrier between its write and its reads,
+// so a buffer is rewritten only after all reads of its previous contents.
 struct Xchg {
   BwdLds& L;
   int par;
   int w, lane;
-  __device__ float2 operator()(float2 mine, int dir, float2 edge) {
+  __device__ float2 operator()(float2 mine, int dir) {
     float2* b = L.x[par];
     par ^= 1;
-    b[w * 64 + lane] = mine;
+    b[(w + 1) * 64 + lane] = mine;
     __syncthreads();
-    const int nw = w + dir;
-    return (nw >= 0 && nw < kBwdBY) ? b[nw * 64 + lane] : edge;
+    return b[(w + 1 + dir) * 64 + lane];
   }
 };
+
+// Zero rows of the exchange buffers (before the first exchange's barrier).
+__device__ __forceinline__ void zero_exchange_edges(BwdLds& L) {
+  if (threadIdx.x < 64) {
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+      L.x[par][threadIdx.x] = make_float2(0.f, 0.f);
+      L.x[par][(kBwdBY + 1) * 64 + threadIdx.x] = make_float2(0.f, 0.f);
+    }
+  }
+}
 
 using Rows = float[kBwdRY][2];
 
@@ -101,7 +117,7 @@ template <bool EDGE>
 __device__ __forceinline__ void gdiff(const Rows& q, Xchg& X, const EdgeMasks& em, Rows& g0,
                                       Rows& g1) {
   constexpr int RY = kBwdRY;
-  const float2 below = X(make_float2(q[0][0], q[0][1]), +1, make_float2(q[RY - 1][0], q[RY - 1][1]));
+  const float2 below = X(make_float2(q[0][0], q[0][1]), +1);
 #pragma unroll
   for (int j = 0; j < RY; ++j) {
     const float dn0 = j + 1 < RY ? q[j + 1][0] : below.x;
@@ -127,7 +143,7 @@ __device__ __forceinline__ void gdiff(const Rows& q, Xchg& X, const EdgeMasks& e
 // (img) or registers.
 __device__ __forceinline__ void gtdiv(const Rows& a0, const Rows& a1, Xchg& X, Rows& out) {
   constexpr int RY = kBwdRY;
-  const float2 above = X(make_float2(a0[RY - 1][0], a0[RY - 1][1]), -1, make_float2(0.f, 0.f));
+  const float2 above = X(make_float2(a0[RY - 1][0], a0[RY - 1][1]), -1);
 #pragma unroll
   for (int j = 0; j < RY; ++j) {
     const float up0 = j > 0 ? a0[j - 1][0] : above.x;
@@ -348,6 +364,7 @@ template <int D>
 __global__ void __launch_bounds__(64 * kBwdBY) k_tv_bwd(BwdArgs a) {
   __shared__ BwdLds L;
   const int tile = blockIdx.x, t = blockIdx.y;
+  zero_exchange_edges(L);
   bwd_tile<D, true>(a, tile, t, L);
   bwd_tile<D, false>(a, tile, t, L);
 }
