@@ -37,11 +37,13 @@ using namespace fused;
 
 constexpr int kMaxBatchDepth = 6;
 
+// 8-wave tiles of depth 5 are held to 128 VGPRs, two workgroups per CU (they
+// would take 132; depth 4 needs 125 on its own).
 template <int D, int BY, int MR, int MODE>
-__global__ void __launch_bounds__(64 * BY) k_fused_batch(const FusedArgs* __restrict__ descs,
-                                                        const int2* __restrict__ queue,
-                                                        const int* __restrict__ qoff,
-                                                        int* counters) {
+__global__ void __launch_bounds__(64 * BY)
+__attribute__((amdgpu_waves_per_eu(BY == 8 && D == 5 ? 4 : 1)))
+k_fused_batch(const FusedArgs* __restrict__ descs, const int2* __restrict__ queue,
+              const int* __restrict__ qoff, int* counters) {
   __shared__ FusedLds<D, BY, MR> sh;
   __shared__ int s_item;
   const int home = blockIdx.x & 7;

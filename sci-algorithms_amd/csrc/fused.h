@@ -645,10 +645,14 @@ __host__ __device__ inline bool tile_inside(int tile, int tiles_x, int H, int W)
 // One launch covers interior and border tiles; the two bodies are separate
 // instantiations (the interior one has no masks or bounds checks).  PART
 // restricts a launch to one class (1: interior, 2: border; 0: both), so that
-// the interior kernel is allocated registers for its own body only.
+// the interior kernel is allocated registers for its own body only.  With the
+// per-iteration error (IQA) the 8-wave kernel is held to 128 VGPRs, the limit
+// for two workgroups per CU (it would take 130-133; with one workgroup per CU
+// a 2048x2048x20 solve takes 48 ms instead of 32).
 template <int D, int BY, int MR, int MODE, bool IQA, bool X1 = false, int PART = 0,
           bool DMA = false>
-__global__ void __launch_bounds__(64 * BY) k_fused(FusedArgs a) {
+__global__ void __launch_bounds__(64 * BY)
+__attribute__((amdgpu_waves_per_eu(BY == 8 && D <= 4 && IQA ? 4 : 1))) k_fused(FusedArgs a) {
   __shared__ FusedLds<D, BY, MR, X1, DMA> sh;
   // Work item -> (tile, frame part); the parts of a tile are adjacent, so
   // they run on the same XCD.
