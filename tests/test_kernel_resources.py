@@ -34,8 +34,12 @@ def _kernels(tmp_path):
             pytest.skip("%s is not installed" % tool)
     if shutil.which("objcopy") is None:
         pytest.skip("objcopy is not installed")
+    # objcopy rewrites the file it reads: give it a copy, never the library
+    # that this process (and the other tests) have mapped.
+    copy = tmp_path / "libscigap.so"
+    shutil.copyfile(lib, copy)
     fat = tmp_path / "fatbin.bin"
-    subprocess.run(["objcopy", "--dump-section", ".hip_fatbin=%s" % fat, lib], check=True,
+    subprocess.run(["objcopy", "--dump-section", ".hip_fatbin=%s" % fat, str(copy)], check=True,
                    capture_output=True)
     data = fat.read_bytes()
     starts = [m.start() for m in re.finditer(re.escape(MAGIC), data)]
