@@ -620,15 +620,20 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int tile,
 // frame loop of one tile is split over workgroups to even out the last
 // round of workgroups on the CUs; their partial yb planes meet here.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_part_epilogue(FusedArgs a) {
+__device__ __forceinline__ void part_epilogue(const FusedArgs& a, int first, int stride) {
   const int N = a.H * a.W;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+  for (int i = first; i < N; i += stride) {
     double sum = a.yb_part[i];
     for (int f = 1; f < a.fsplit; ++f) sum += a.yb_part[size_t(f) * N + i];
     const float yb = float(sum);
     if (a.yb) a.yb[i] = yb;
     if (a.epilogue) epilogue_pixel<MODE>(a, unsigned(i), yb);
   }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_part_epilogue(FusedArgs a) {
+  part_epilogue<MODE>(a, int(blockIdx.x * blockDim.x + threadIdx.x), int(gridDim.x * blockDim.x));
 }
 
 // Whether a tile's load region lies entirely in the image (mirrors TilePlace).
@@ -732,6 +737,21 @@ constexpr int kDecideThreads = 1024;
 template <int D>
 __global__ void __launch_bounds__(kDecideThreads) k_decide(FusedArgs a) {
   decide_frame<D, kDecideThreads>(a, blockIdx.x);
+}
+
+// The decision and, for a main pass split into frame parts, the parts' yb sum
+// and residual epilogue in one launch: workgroups [0, T) decide one frame
+// each, the rest run the epilogue over the pixels (the two read different
+// results of the main pass, and the repair after them needs both).  One
+// launch less per outer iteration where launches dominate (small problems).
+template <int D, int MODE>
+__global__ void __launch_bounds__(kDecideThreads) k_decide_epilogue(FusedArgs a) {
+  if (blockIdx.x < unsigned(a.T)) {  // workgroup-uniform
+    decide_frame<D, kDecideThreads>(a, blockIdx.x);
+    return;
+  }
+  const int b = int(blockIdx.x) - a.T, nb = int(gridDim.x) - a.T;
+  part_epilogue<MODE>(a, b * kDecideThreads + int(threadIdx.x), nb * kDecideThreads);
 }
 
 // Early-exit repair (does work only when some frame stopped early; `stop`

@@ -137,8 +137,11 @@ int pick_parts(int tiles, int T, int slots) {
   return best;
 }
 
+// With `fold_parts`, a main pass split into frame parts leaves the parts'
+// epilogue to the caller's decision launch (k_decide_epilogue) and reports
+// the split there; otherwise it runs k_part_epilogue itself.
 template <int D, int BY, int MODE, bool IQA, int MR = 1>
-int launch_main(const FusedArgs& a0, hipStream_t s, int engine) {
+int launch_main(const FusedArgs& a0, hipStream_t s, int engine, int* fold_parts = nullptr) {
   FusedArgs a = a0;
   const unsigned grid = set_tiles<D, BY, MR>(a);
   if (engine == 0) engine = kDefaultEngine;
@@ -154,9 +157,13 @@ int launch_main(const FusedArgs& a0, hipStream_t s, int engine) {
     ProfScope ps(MODE == TV_ONLY ? "tv_fused" : "fused_iteration", s);
     launch_kernel<D, BY, MODE, IQA, false, 0, false, MR>(a, unsigned(a.per_xcd * 8), s);
     if constexpr (MODE != TV_ONLY) {
-      const int64_t N = int64_t(a.H) * a.W;
-      const unsigned g = unsigned(std::min<int64_t>((N + 255) / 256, 4096));
-      k_part_epilogue<MODE><<<g, 256, 0, s>>>(a);
+      if (fold_parts) {
+        *fold_parts = parts;
+      } else {
+        const int64_t N = int64_t(a.H) * a.W;
+        const unsigned g = unsigned(std::min<int64_t>((N + 255) / 256, 4096));
+        k_part_epilogue<MODE><<<g, 256, 0, s>>>(a);
+      }
     }
   } else {
     ProfScope ps(MODE == TV_ONLY ? "tv_fused" : "fused_iteration", s);
@@ -190,18 +197,18 @@ int launch_main(const FusedArgs& a0, hipStream_t s, int engine) {
 }
 
 template <int D, int BY, int MR = 1>
-int launch_mode(const FusedArgs& a, hipStream_t s, int engine) {
+int launch_mode(const FusedArgs& a, hipStream_t s, int engine, int* fold_parts = nullptr) {
   const bool iqa = a.truth != nullptr;
   if (a.mode == GAP_ACC)
-    return iqa ? launch_main<D, BY, GAP_ACC, true, MR>(a, s, engine)
-               : launch_main<D, BY, GAP_ACC, false, MR>(a, s, engine);
+    return iqa ? launch_main<D, BY, GAP_ACC, true, MR>(a, s, engine, fold_parts)
+               : launch_main<D, BY, GAP_ACC, false, MR>(a, s, engine, fold_parts);
   if (a.mode == GAP_PLAIN)
-    return iqa ? launch_main<D, BY, GAP_PLAIN, true, MR>(a, s, engine)
-               : launch_main<D, BY, GAP_PLAIN, false, MR>(a, s, engine);
+    return iqa ? launch_main<D, BY, GAP_PLAIN, true, MR>(a, s, engine, fold_parts)
+               : launch_main<D, BY, GAP_PLAIN, false, MR>(a, s, engine, fold_parts);
   if (a.mode == ADMM)
-    return iqa ? launch_main<D, BY, ADMM, true, MR>(a, s, engine)
-               : launch_main<D, BY, ADMM, false, MR>(a, s, engine);
-  return launch_main<D, BY, TV_ONLY, false, MR>(a, s, engine);
+    return iqa ? launch_main<D, BY, ADMM, true, MR>(a, s, engine, fold_parts)
+               : launch_main<D, BY, ADMM, false, MR>(a, s, engine, fold_parts);
+  return launch_main<D, BY, TV_ONLY, false, MR>(a, s, engine, fold_parts);
 }
 
 // Tiles of the main launch for (D, BY, MR): the energy slots' row length.
@@ -231,16 +238,20 @@ int launch_fused(const FusedArgs& a00, hipStream_t s, int tile_rows, int engine)
       return fail(SCIGAP_EWORKSPACE, "energy slots hold %d tiles, the launch has %d",
                   a0.eslot_stride, a0.eslot_tiles);
   }
+  // When a decision follows, a main pass split into frame parts leaves its
+  // parts' epilogue to the decision launch (parts > 1 then).
+  int parts = 1;
+  int* const fold = a0.stop && a0.eslot ? &parts : nullptr;
   int rc;
   if constexpr (D == 4) {
     // 8 waves measured fastest at 2048x2048x20 (419 us against 509 us for
     // 12 and 478 us for 16 per outer iteration): at about 110 VGPRs two
     // 8-wave workgroups share a CU, so one hides the other's barriers.
-    rc = tile_rows == 12   ? launch_mode<4, 12>(a0, s, engine)
-         : tile_rows == 16 ? launch_mode<4, 16>(a0, s, engine)
-                           : launch_mode<4, 8>(a0, s, engine);
+    rc = tile_rows == 12   ? launch_mode<4, 12>(a0, s, engine, fold)
+         : tile_rows == 16 ? launch_mode<4, 16>(a0, s, engine, fold)
+                           : launch_mode<4, 8>(a0, s, engine, fold);
   } else {
-    rc = launch_mode<D, BY>(a0, s, engine);
+    rc = launch_mode<D, BY>(a0, s, engine, fold);
   }
   if (rc) return rc;
   // No stop array: the caller runs without the energy test (eps = 0), so no
@@ -252,7 +263,23 @@ int launch_fused(const FusedArgs& a00, hipStream_t s, int tile_rows, int engine)
   const unsigned grid = set_tiles<D, FBY, MR>(a);
   {
     ProfScope ps("fused_decide", s);
-    k_decide<D><<<unsigned(a.T), kDecideThreads, 0, s>>>(a);
+    if (parts > 1) {
+      FusedArgs e = a0;
+      e.fsplit = parts;
+      const int64_t N = int64_t(a.H) * a.W;
+      const unsigned g =
+          unsigned(std::min<int64_t>((N + kDecideThreads - 1) / kDecideThreads, 1024));
+      const unsigned blocks = unsigned(a.T) + g;
+      switch (a.mode) {
+        case GAP_ACC: k_decide_epilogue<D, GAP_ACC><<<blocks, kDecideThreads, 0, s>>>(e); break;
+        case GAP_PLAIN:
+          k_decide_epilogue<D, GAP_PLAIN><<<blocks, kDecideThreads, 0, s>>>(e);
+          break;
+        default: k_decide_epilogue<D, ADMM><<<blocks, kDecideThreads, 0, s>>>(e); break;
+      }
+    } else {
+      k_decide<D><<<unsigned(a.T), kDecideThreads, 0, s>>>(a);
+    }
     SCIGAP_LAUNCH_CHECK("early-exit decision launch");
   }
   {
